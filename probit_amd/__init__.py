@@ -1,0 +1,22 @@
+"""probit_amd: MI355X (gfx950) HIP implementation of probit's GP posterior path.
+
+Drop-in for ``probit.approximators.LaplaceGP`` / ``VBGP``, the likelihood
+utilities of ``probit.utilities`` and the mlkernels kernel expressions the
+reference's priors use.  All compute runs in ``libprobit_amd.so`` (hand-written
+HIP kernels behind a C ABI, ``include/probit_amd.h``); build it with
+``python -m probit_amd.build``.
+"""
+
+from .approximators import Approximator, LaplaceGP, VBGP
+from .kernels import EQ, Kernel, Matern12
+from .utilities import (BOUNDS, CutpointValueError, InvalidKernel, check_cutpoints, check_data,
+                        grad_log_probit_likelihood, h, hessian_log_probit_likelihood,
+                        log_gaussian_likelihood, log_probit_likelihood,
+                        probit_predictive_distributions)
+
+__all__ = [
+    "Approximator", "LaplaceGP", "VBGP", "EQ", "Matern12", "Kernel", "BOUNDS",
+    "CutpointValueError", "InvalidKernel", "check_cutpoints", "check_data",
+    "grad_log_probit_likelihood", "hessian_log_probit_likelihood", "h",
+    "log_gaussian_likelihood", "log_probit_likelihood", "probit_predictive_distributions",
+]

@@ -1,0 +1,480 @@
+"""LaplaceGP / VBGP with the reference's API (probit/approximators.py), on HIP.
+
+Signatures follow the reference:
+  * ``__init__(data, prior, log_likelihood, grad_log_likelihood=None,
+    hessian_log_likelihood=None, tolerance=1e-5)`` (probit/approximators.py:55-63)
+  * ``approximate_posterior(parameters) -> (weight, precision)`` (:204-210)
+  * ``predict(X_test, parameters, weight, precision) -> (mean, variance)`` (:154-180)
+  * ``predict_covariance(...)`` (:182-197), ``objective()`` (:248-263, :316-330),
+    ``value_and_grad()`` (:132-134), ``weight``, ``precision``, ``construct``.
+
+Extra keyword-only arguments choose the factor precision ("fp64", "fp32",
+"bf16"), the Newton iteration cap and the number of FP64 refinement steps.
+All compute runs in libprobit_amd (HIP, gfx950); torch supplies device
+memory, streams and the autograd graph around the evidence.  Numerical
+failures surface as NaN (as JAX would produce) plus ``last_result.status``.
+"""
+
+from __future__ import annotations
+
+import warnings
+from abc import ABC, abstractmethod
+
+import numpy as np
+import torch
+
+from . import _lib
+from .kernels import Kernel
+from .utilities import (InvalidKernel, grad_log_probit_likelihood,
+                        hessian_log_probit_likelihood, likelihood_struct,
+                        log_gaussian_likelihood, log_probit_likelihood)
+
+__all__ = ["Approximator", "LaplaceGP", "VBGP"]
+
+
+def _resolve_likelihood(log_likelihood, grad_log_likelihood, hessian_log_likelihood) -> int:
+    """Map the reference's likelihood callables to a fused-kernel ID."""
+    if log_likelihood is log_gaussian_likelihood:
+        if grad_log_likelihood is not None or hessian_log_likelihood is not None:
+            raise NotImplementedError("custom derivatives of the Gaussian likelihood")
+        return _lib.LIK_GAUSSIAN
+    if log_likelihood is log_probit_likelihood:
+        if grad_log_likelihood is None and hessian_log_likelihood is None:
+            return _lib.LIK_PROBIT
+        if (grad_log_likelihood is grad_log_probit_likelihood
+                and hessian_log_likelihood is hessian_log_probit_likelihood):
+            return _lib.LIK_PROBIT_CLOSED
+    raise NotImplementedError(
+        "no fused HIP kernel for this likelihood: use probit_amd.utilities."
+        "log_gaussian_likelihood or log_probit_likelihood (optionally with "
+        "grad_/hessian_log_probit_likelihood)")
+
+
+def _flatten(tree):
+    if isinstance(tree, (tuple, list)):
+        leaves, specs = [], []
+        for t in tree:
+            lv, sp = _flatten(t)
+            leaves += lv
+            specs.append(sp)
+        return leaves, (type(tree), specs)
+    return [tree], None
+
+
+def _unflatten(spec, leaves):
+    it = iter(leaves)
+
+    def rec(sp):
+        if sp is None:
+            return next(it)
+        typ, subs = sp
+        return typ(rec(s) for s in subs)
+
+    return rec(spec)
+
+
+class Approximator(ABC):
+    """Base class: data upload, parameter packing, workspaces, predict."""
+
+    @abstractmethod
+    def __repr__(self):
+        """Name of the approximator."""
+
+    def __init__(self, data, prior, log_likelihood, grad_log_likelihood=None,
+                 hessian_log_likelihood=None, tolerance=1e-5, *, precision_mode="fp64",
+                 maxiter=100, refine=None, device=None):
+        self.tolerance = float(tolerance)
+        self.prior = prior
+        self.lik_kind = _resolve_likelihood(log_likelihood, grad_log_likelihood,
+                                            hessian_log_likelihood)
+        self.log_likelihood = log_likelihood
+        self.grad_log_likelihood = grad_log_likelihood
+        self.hessian_log_likelihood = hessian_log_likelihood
+        if precision_mode not in _lib.PRECISIONS:
+            raise ValueError(f"precision_mode must be one of {list(_lib.PRECISIONS)}")
+        self.precision_mode = precision_mode
+        self._prec = _lib.PRECISIONS[precision_mode]
+        self.maxiter = int(maxiter)
+        self.refine = -1 if refine is None else int(refine)
+        X_train, y_train = data
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        X = torch.as_tensor(np.asarray(X_train) if not isinstance(X_train, torch.Tensor)
+                            else X_train).detach().to(torch.float64)
+        if X.dim() == 1:
+            X = X[:, None]
+        self.N, self.D = X.shape
+        y = torch.as_tensor(np.asarray(y_train) if not isinstance(y_train, torch.Tensor)
+                            else y_train).detach().reshape(-1).to(torch.float64)
+        if y.shape[0] != self.N:
+            raise ValueError("X and y disagree on N")
+        self._X = X.to(self.device).contiguous()
+        self._y = y.to(self.device).contiguous()
+        self.data = data
+        self._ws = None
+        self._pws = None
+        self._gws = None
+        self._state = None
+        self.last_result = None
+
+    # ---- packing ----------------------------------------------------------
+    def _kernel(self, prior_parameters) -> Kernel:
+        k = self.prior(prior_parameters)
+        if not isinstance(k, Kernel):
+            raise InvalidKernel(k)
+        return k
+
+    def _structs(self, parameters):
+        kern = self._kernel(parameters[0])
+        ks = kern.to_struct(self.D)
+        ls = likelihood_struct(self.lik_kind, parameters[1])
+        return ks, ls
+
+    @staticmethod
+    def _key(ks, ls):
+        return (ks.kind, ks.variance, tuple(ks.lengthscale), tuple(ks.period), ls.kind,
+                ls.noise_std, ls.J, tuple(ls.cutpoints))
+
+    def _problem(self, weight=None, precision=None, latent=None) -> _lib.ProblemT:
+        p = _lib.ProblemT()
+        p.N = self.N
+        p.D = self.D
+        p.X = self._X.data_ptr()
+        p.y = self._y.data_ptr()
+        p.weight = _lib.ptr(weight)
+        p.precision = _lib.ptr(precision)
+        p.latent = _lib.ptr(latent)
+        return p
+
+    def _options(self) -> _lib.FitOptionsT:
+        o = _lib.FitOptionsT()
+        o.tolerance = self.tolerance
+        o.maxiter = self.maxiter
+        o.precision = self._prec
+        o.refine = self.refine
+        return o
+
+    def _workspace(self) -> torch.Tensor:
+        lib = _lib.lib()
+        need = lib.pgp_workspace_size(self.N, self.D, self._prec)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = _lib.workspace(need, self.device)
+        return self._ws
+
+    def _grad_workspace(self) -> torch.Tensor:
+        lib = _lib.lib()
+        need = lib.pgp_grad_workspace_size(self.N, self.D)
+        if self._gws is None or self._gws.numel() < need:
+            self._gws = _lib.workspace(need, self.device)
+        return self._gws
+
+    def _stream(self):
+        return _lib.stream_handle(self.device)
+
+    def _vec(self):
+        return torch.empty(self.N, dtype=torch.float64, device=self.device)
+
+    # ---- resident-state bookkeeping -----------------------------------------
+    def _remember(self, key, weight, precision, result):
+        self._state = dict(key=key, weight=weight, precision=precision,
+                           wv=(weight.data_ptr(), weight._version),
+                           pv=(precision.data_ptr(), precision._version))
+        self.last_result = result
+
+    def _state_matches(self, key, weight, precision) -> bool:
+        st = self._state
+        if st is None or st["key"] != key:
+            return False
+        if not (isinstance(weight, torch.Tensor) and isinstance(precision, torch.Tensor)):
+            return False
+        return ((weight.data_ptr(), weight._version) == st["wv"]
+                and (precision.data_ptr(), precision._version) == st["pv"])
+
+    def _as_device_vec(self, v) -> torch.Tensor:
+        t = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))
+        return t.detach().to(torch.float64).reshape(-1).to(self.device).contiguous()
+
+    def _set_state(self, ks, ls, weight, precision):
+        """Factor B at a caller-given (weight, precision) (predict's input)."""
+        lib = _lib.lib()
+        ws = self._workspace()
+        w = self._as_device_vec(weight)
+        W = None if precision is None else self._as_device_vec(precision)
+        Wout = self._vec() if W is None else None
+        fout = self._vec()
+        res = _lib.FitResultT()
+        rc = lib.pgp_laplace_set_state(self._problem(None, Wout, fout), ks, ls, self._prec,
+                                       w.data_ptr(), _lib.ptr(W), res, ws.data_ptr(), ws.numel(),
+                                       self._stream())
+        _lib.check(rc, "pgp_laplace_set_state")
+        Wt = W if W is not None else Wout
+        self._remember(self._key(ks, ls), w, Wt, res)
+        return w, Wt, fout, res
+
+    # ---- public API ---------------------------------------------------------
+    def predict(self, X_test, parameters, weight, precision):
+        """Latent predictive mean and variance (probit/approximators.py:154-180)."""
+        lib = _lib.lib()
+        ks, ls = self._structs(parameters)
+        key = self._key(ks, ls)
+        if not self._state_matches(key, weight, precision):
+            _, Wt, _, res = self._set_state(ks, ls, weight, precision)
+            if res.status & _lib.INFO_NEGATIVE_W:
+                warnings.warn("precision has negative entries: the B = I + W^1/2 K W^1/2 form "
+                              "is undefined there; predictive variances are not valid")
+        Xt = torch.as_tensor(np.asarray(X_test) if not isinstance(X_test, torch.Tensor)
+                             else X_test).detach().to(torch.float64)
+        if Xt.dim() == 1:
+            Xt = Xt[:, None]
+        Xt = Xt.to(self.device).contiguous()
+        M = Xt.shape[0]
+        if Xt.shape[1] != self.D:
+            raise ValueError(f"X_test has D={Xt.shape[1]}, expected {self.D}")
+        need = lib.pgp_predict_workspace_size(self.N, M, self._prec)
+        if self._pws is None or self._pws.numel() < need:
+            self._pws = _lib.workspace(need, self.device)
+        mean = torch.empty(M, dtype=torch.float64, device=self.device)
+        var = torch.empty(M, dtype=torch.float64, device=self.device)
+        ws = self._workspace()
+        rc = lib.pgp_predict(ks, Xt.data_ptr(), M, mean.data_ptr(), var.data_ptr(),
+                             ws.data_ptr(), ws.numel(), self._pws.data_ptr(), self._pws.numel(),
+                             self._stream())
+        _lib.check(rc, "pgp_predict")
+        return mean, var
+
+    def predict_covariance(self, X_test, parameters, weight, precision):
+        """Predictive covariance Kss - Kfs^T (K + W^-1)^-1 Kfs (probit/approximators.py:182-197)."""
+        raise NotImplementedError("predict_covariance is not provided by this build")
+
+    def posterior_mean(self, weight):
+        """The reference's posterior_mean (probit/approximators.py:199-202) reads an
+        undefined `parameters` and cannot run; use precision(weight, parameters)."""
+        raise NotImplementedError("posterior_mean needs parameters: use precision()")
+
+    def approximate_posterior(self, parameters):
+        """(weight, precision) of the approximate posterior (probit/approximators.py:204-210)."""
+        return self._fit(parameters)
+
+    def value_and_grad(self):
+        """Value and gradient of the objective at the fixed point
+        (probit/approximators.py:132-134).  Returns fn(parameters) ->
+        (value, grads) with grads shaped like parameters (float64 NumPy)."""
+        obj = self.objective()
+
+        def fn(parameters):
+            leaves, spec = _flatten(parameters)
+            tl = [torch.as_tensor(np.asarray(v, dtype=np.float64) if not isinstance(v, torch.Tensor)
+                                  else v.detach().cpu().to(torch.float64)).clone().requires_grad_(True)
+                  for v in leaves]
+            val = obj(_unflatten(spec, tl))
+            grads = torch.autograd.grad(val, tl, allow_unused=True)
+            out = [np.zeros(t.shape) if g is None else g.detach().cpu().numpy()
+                   for t, g in zip(tl, grads)]
+            return float(val.detach()), _unflatten(spec, out)
+
+        return fn
+
+    @abstractmethod
+    def _fit(self, parameters):
+        """Run the fixed point; return (weight, precision)."""
+
+    @abstractmethod
+    def construct(self):
+        """The fixed-point map (parameters, weight) -> weight'."""
+
+    @abstractmethod
+    def objective(self):
+        """The objective as fn(parameters) -> scalar tensor."""
+
+    @abstractmethod
+    def weight(self, parameters):
+        """The weight a (f = K a)."""
+
+    @abstractmethod
+    def precision(self, weight, parameters):
+        """(precision, posterior_mean)."""
+
+
+class _LaplaceEvidence(torch.autograd.Function):
+    """objective_LA with its hand-written adjoint (pgp_laplace_grad)."""
+
+    @staticmethod
+    def forward(ctx, gp, parameters, var, ls, per, noise, cut):
+        ks, lsd = gp._structs(parameters)
+        res = gp._run_fit(ks, lsd)[2]
+        ctx.gp = gp
+        ctx.key = gp._key(ks, lsd)
+        ctx.structs = (ks, lsd)
+        ctx.parameters = parameters
+        ctx.D = gp.D
+        ctx.has_per = per is not None
+        ctx.has_cut = cut is not None
+        ctx.devs = [t.device if t is not None else None for t in (var, ls, per, noise, cut)]
+        return torch.tensor(res.objective, dtype=torch.float64, device=var.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        gp = ctx.gp
+        ks, lsd = ctx.structs
+        if gp._state is None or gp._state["key"] != ctx.key:
+            gp._run_fit(ks, lsd)
+        vec = gp._grad(ks, lsd)
+        D = ctx.D
+        g = g.detach().cpu().double()
+        o = 0
+        gvar = torch.tensor(vec[o]) * g
+        o += 1
+        gls = torch.tensor(vec[o:o + D]) * g
+        o += D
+        gper = torch.tensor(vec[o:o + D]) * g if ctx.has_per else None
+        o += D
+        gnoise = torch.tensor(vec[o]) * g
+        o += 1
+        gcut = None
+        if ctx.has_cut:
+            J = lsd.J
+            gcut = torch.tensor(vec[o:o + J + 1]) * g
+        outs = [gvar, gls, gper, gnoise, gcut]
+        outs = [None if t is None else t.to(d) for t, d in zip(outs, ctx.devs)]
+        return (None, None, *outs)
+
+
+class LaplaceGP(Approximator):
+    """Laplace approximation (probit/approximators.py:213-277)."""
+
+    def __repr__(self):
+        return "LaplaceGP"
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+
+    def _run_fit(self, ks, ls):
+        lib = _lib.lib()
+        ws = self._workspace()
+        weight, precision = self._vec(), self._vec()
+        res = _lib.FitResultT()
+        rc = lib.pgp_laplace_fit(self._problem(weight, precision), ks, ls, self._options(), res,
+                                 ws.data_ptr(), ws.numel(), self._stream())
+        _lib.check(rc, "pgp_laplace_fit")
+        self._remember(self._key(ks, ls), weight, precision, res)
+        if res.status & _lib.INFO_MAXITER:
+            warnings.warn(f"Newton iteration stopped at maxiter={self.maxiter} "
+                          f"(step {res.step_norm:.3e} > tol {self.tolerance:.1e})")
+        return weight, precision, res
+
+    def _fit(self, parameters):
+        ks, ls = self._structs(parameters)
+        w, p, _ = self._run_fit(ks, ls)
+        return w, p
+
+    def _grad(self, ks, ls):
+        lib = _lib.lib()
+        ws = self._workspace()
+        gws = self._grad_workspace()
+        out = (np.zeros(2 + 2 * self.D + _lib.MAX_CUT)).astype(np.float64)
+        rc = lib.pgp_laplace_grad(ks, ls, out.ctypes.data, ws.data_ptr(), ws.numel(),
+                                  gws.data_ptr(), gws.numel(), self._stream())
+        _lib.check(rc, "pgp_laplace_grad")
+        return out
+
+    def construct(self):
+        """f_LA: (parameters, weight) -> grad log p(y | K weight)
+        (probit/approximators.py:238-246, probit/implicit/Laplace.py:4-9)."""
+
+        def f(parameters, weight):
+            ks, ls = self._structs(parameters)
+            _, _, latent, _ = self._set_state(ks, ls, weight, None)
+            d1 = torch.empty_like(latent)
+            rc = _lib.lib().pgp_likelihood_eval(ls, latent.data_ptr(), self._y.data_ptr(), self.N,
+                                                None, d1.data_ptr(), None, self._stream())
+            _lib.check(rc, "pgp_likelihood_eval")
+            return d1
+
+        return f
+
+    def objective(self):
+        """-log q(y | X, theta) at the Laplace fixed point
+        (probit/approximators.py:248-263, probit/implicit/Laplace.py:12-30),
+        differentiable through torch autograd via the hand-written adjoint."""
+
+        def obj(parameters):
+            kern = self._kernel(parameters[0])
+            var, ls, per = kern.spec_tensors(self.D)
+            lp = parameters[1]
+            lp = lp if isinstance(lp, (tuple, list)) else (lp,)
+            noise = torch.as_tensor(lp[0], dtype=torch.float64) if not isinstance(
+                lp[0], torch.Tensor) else lp[0].to(torch.float64)
+            cut = None
+            if self.lik_kind != _lib.LIK_GAUSSIAN:
+                c = lp[1]
+                cut = c.to(torch.float64) if isinstance(c, torch.Tensor) else torch.as_tensor(
+                    np.asarray(c, dtype=np.float64))
+            return _LaplaceEvidence.apply(self, parameters, var, ls, per, noise.reshape(()),
+                                          cut)
+
+        return obj
+
+    def weight(self, parameters):
+        """Newton fixed point from zeros (probit/approximators.py:265-269)."""
+        return self._fit(parameters)[0]
+
+    def precision(self, weight, parameters):
+        """(W, f) with f = K weight, W = -d2 log p (probit/approximators.py:271-277)."""
+        ks, ls = self._structs(parameters)
+        _, W, f, _ = self._set_state(ks, ls, weight, None)
+        return W, f
+
+
+class VBGP(Approximator):
+    """Variational Bayes approximation (probit/approximators.py:280-339)."""
+
+    def __repr__(self):
+        return "VBGP"
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+
+    def _run_fit(self, ks, ls):
+        lib = _lib.lib()
+        ws = self._workspace()
+        gws = self._grad_workspace()
+        weight, precision = self._vec(), self._vec()
+        res = _lib.FitResultT()
+        rc = lib.pgp_vb_fit(self._problem(weight, precision), ks, ls, self._options(), res,
+                            ws.data_ptr(), ws.numel(), gws.data_ptr(), gws.numel(),
+                            self._stream())
+        _lib.check(rc, "pgp_vb_fit")
+        self._remember(self._key(ks, ls), weight, precision, res)
+        if res.status & _lib.INFO_MAXITER:
+            warnings.warn(f"fixed-point iteration stopped at maxiter={self.maxiter}")
+        return weight, precision, res
+
+    def _fit(self, parameters):
+        ks, ls = self._structs(parameters)
+        w, p, _ = self._run_fit(ks, ls)
+        return w, p
+
+    def construct(self):
+        raise NotImplementedError("VBGP.construct (f_VB as a standalone map) is not provided")
+
+    def objective(self):
+        """objective_VB (probit/implicit/VB.py:19-40) at the VB fixed point (value only)."""
+
+        def obj(parameters):
+            ks, ls = self._structs(parameters)
+            res = self._run_fit(ks, ls)[2]
+            return torch.tensor(res.objective, dtype=torch.float64)
+
+        return obj
+
+    def weight(self, parameters):
+        return self._fit(parameters)[0]
+
+    def precision(self, weight, parameters):
+        """(1/sigma^2, K weight) (probit/approximators.py:336-339)."""
+        ks, ls = self._structs(parameters)
+        w = self._as_device_vec(weight)
+        _, _, f, _ = self._set_state(ks, ls, w, torch.full((self.N,), 1.0 / ls.noise_std**2,
+                                                           dtype=torch.float64,
+                                                           device=self.device))
+        return torch.full((self.N,), 1.0 / ls.noise_std**2, dtype=torch.float64,
+                          device=self.device), f

@@ -1,0 +1,965 @@
+// Host-side drivers and the extern "C" ABI of libprobit_amd.
+//
+// Control flow mirrors the reference call stacks:
+//   pgp_laplace_fit   <- LaplaceGP.approximate_posterior (probit/approximators.py:204-210)
+//                        Newton loop of newton_solver / fwd_solver (probit/implicit/solvers.py:7-25)
+//   pgp_predict       <- Approximator.predict (probit/approximators.py:154-180)
+//   pgp_laplace_grad  <- value_and_grad(objective) (probit/approximators.py:132-134, 248-263)
+//   pgp_vb_fit        <- VBGP.approximate_posterior + objective_VB (probit/approximators.py:306-339)
+// The stopping rule is jaxopt's: iterate until ||z_{k+1} - z_k||_2 <= tol or
+// maxiter, returning z_{k+1}.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "pgp_internal.hpp"
+
+namespace pgp {
+
+// grad.hip
+void launch_sym_from_inv(const Batch& b, int prec, int use_scale, hipStream_t s);
+void launch_fill_rhs(const Batch& b, int prec, int mode, hipStream_t s);
+void launch_sigma_diag(const Batch& b, int prec, hipStream_t s);
+void launch_grad_trace(const Batch& b, double* partials, double* out, hipStream_t s);
+void launch_vb_trace(const Batch& b, double* out, hipStream_t s);
+int grad_npar();
+int grad_maxd();
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) return fail(-100, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(-101, "kernel launch: %s", hipGetErrorString(e));
+  return 0;
+}
+
+static inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+static inline int64_t align256(int64_t x) { return round_up(x, 256); }
+static inline int elem_size(int prec) { return prec == PGP_PREC_FP64 ? 8 : 4; }
+
+// ---------------------------------------------------------------------------
+// workspace layouts
+// ---------------------------------------------------------------------------
+constexpr int64_t OFF_PROB = 0;
+constexpr int64_t OFF_ARR = 2048;   // pointer array (self) for single-problem launches
+constexpr int64_t OFF_INFO = 2112;  // int32[4]
+constexpr int64_t OFF_RED = 2176;   // double[NRED]
+constexpr int64_t OFF_GATHER = 2432;  // double[8] scratch for reductions
+constexpr int64_t HEADER = 4096;
+
+struct WsLayout {
+  int64_t Np, K, L, logdiag, diagval, dinv, vec, total;
+};
+
+static WsLayout ws_layout(int64_t N, int prec) {
+  WsLayout w;
+  w.Np = round_up(N, TILE);
+  int64_t off = HEADER;
+  w.K = off;
+  off += align256(w.Np * w.Np * 8);
+  w.L = off;
+  off += align256(w.Np * w.Np * elem_size(prec));
+  w.logdiag = off;
+  off += align256(w.Np * 8);
+  w.diagval = off;
+  off += align256(w.Np * 8);
+  w.dinv = off;
+  off += align256(w.Np * TILE * 8);
+  w.vec = off;
+  off += align256((int64_t)NVEC * w.Np * 8);
+  w.total = off;
+  return w;
+}
+
+struct PredLayout {
+  int64_t Mp, S, mupart, total;
+};
+static PredLayout pred_layout(int64_t Np, int64_t M, int prec) {
+  PredLayout p;
+  p.Mp = round_up(M, TILE);
+  int64_t off = 0;
+  p.S = off;
+  off += align256(Np * p.Mp * elem_size(prec));
+  p.mupart = off;
+  off += align256((Np / TILE) * p.Mp * 8);
+  p.total = off;
+  return p;
+}
+
+struct GradLayout {
+  int64_t C, G2, partials, out, total;
+};
+static GradLayout grad_layout(int64_t Np, int prec) {
+  GradLayout g;
+  const int64_t T = Np / TILE;
+  int64_t off = 0;
+  g.C = off;
+  off += align256(Np * Np * 8);
+  g.G2 = off;
+  off += align256(Np * Np * elem_size(prec));
+  g.partials = off;
+  off += align256(T * (T + 1) / 2 * grad_npar() * 8);
+  g.out = off;
+  off += align256(256 * 8);
+  g.total = off;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// device-side descriptor and pointer-array writes (kernel arguments carry the
+// host values, so no host buffer has to outlive the call)
+// ---------------------------------------------------------------------------
+__global__ void k_write_prob(Prob* dst, Prob val, int reset) {
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(&val);
+  uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+  const int nw = (int)(sizeof(Prob) / 8);
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) d[i] = src[i];
+  if (reset) {
+    if (threadIdx.x < 4) val.info[threadIdx.x] = 0;
+    if (threadIdx.x < NRED) val.red[threadIdx.x] = 0.0;
+  }
+}
+
+constexpr int PTRPACK = 448;
+struct PtrPack {
+  Prob* p[PTRPACK];
+};
+__global__ void k_write_ptrs(Prob** dst, int n, PtrPack pk) {
+  const int i = threadIdx.x + blockIdx.x * blockDim.x;
+  if (i < n) dst[i] = pk.p[i];
+}
+
+static_assert(sizeof(Prob) % 8 == 0, "Prob must be 8-byte granular");
+static_assert(sizeof(Prob) <= 2048, "Prob must fit the workspace header");
+
+static void write_prob(char* ws, const Prob& p, int reset, hipStream_t s) {
+  hipLaunchKernelGGL(k_write_prob, dim3(1), dim3(256), 0, s, (Prob*)(ws + OFF_PROB), p, reset);
+}
+
+static void write_ptrs(Prob** dst, const std::vector<Prob*>& v, hipStream_t s) {
+  for (size_t b = 0; b < v.size(); b += PTRPACK) {
+    PtrPack pk;
+    const int n = (int)std::min<size_t>(PTRPACK, v.size() - b);
+    for (int i = 0; i < n; ++i) pk.p[i] = v[b + i];
+    hipLaunchKernelGGL(k_write_ptrs, dim3((n + 255) / 256), dim3(256), 0, s, dst + b, n, pk);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// descriptor construction from the ABI structs
+// ---------------------------------------------------------------------------
+static int validate(const pgp_problem_t* pr, const pgp_kernel_t* k, const pgp_likelihood_t* l) {
+  if (!pr) return fail(-1, "problem is NULL");
+  if (pr->N <= 0) return fail(-1, "problem N must be positive (got %lld)", (long long)pr->N);
+  if (pr->D <= 0 || pr->D > PGP_MAX_DIM)
+    return fail(-1, "problem D must be in [1, %d] (got %d)", PGP_MAX_DIM, pr->D);
+  if (!pr->X || !pr->y) return fail(-1, "problem X / y must be device pointers");
+  if (!k) return fail(-2, "kernel is NULL");
+  if (k->kind != PGP_KERNEL_EQ && k->kind != PGP_KERNEL_MATERN12)
+    return fail(-2, "unknown kernel kind %d", k->kind);
+  if (k->dim != pr->D) return fail(-2, "kernel dim %d != problem D %d", k->dim, pr->D);
+  if (!(k->variance > 0.0)) return fail(-2, "kernel variance must be > 0");
+  for (int d = 0; d < k->dim; ++d) {
+    if (!(k->lengthscale[d] > 0.0)) return fail(-2, "lengthscale[%d] must be > 0", d);
+    if (k->period[d] < 0.0) return fail(-2, "period[%d] must be >= 0", d);
+  }
+  if (l) {
+    if (l->kind < PGP_LIK_GAUSSIAN || l->kind > PGP_LIK_PROBIT_CLOSED)
+      return fail(-3, "unknown likelihood kind %d", l->kind);
+    if (!(l->noise_std > 0.0)) return fail(-3, "noise_std must be > 0");
+    if (l->kind != PGP_LIK_GAUSSIAN) {
+      if (l->J < 2 || l->J + 1 > PGP_MAX_CUT)
+        return fail(-3, "J must be in [2, %d] (got %d)", PGP_MAX_CUT - 1, l->J);
+      if (!(isinf(l->cutpoints[0]) && l->cutpoints[0] < 0) ||
+          !(isinf(l->cutpoints[l->J]) && l->cutpoints[l->J] > 0))
+        return fail(-3, "cutpoints must start at -inf and end at +inf");
+      for (int j = 0; j < l->J; ++j)
+        if (!(l->cutpoints[j] <= l->cutpoints[j + 1]))
+          return fail(-3, "cutpoints must be ascending");
+    }
+  }
+  return 0;
+}
+
+static void fill_kernel(Prob& p, const pgp_kernel_t* k) {
+  p.kern_kind = k->kind;
+  p.variance = k->variance;
+  p.periodic = 0;
+  for (int d = 0; d < PGP_MAX_DIM; ++d) {
+    p.ls[d] = 1.0;
+    p.inv_ls2[d] = 0.0;
+    p.pi_over_per[d] = 0.0;
+    p.per[d] = 0.0;
+  }
+  for (int d = 0; d < k->dim; ++d) {
+    p.ls[d] = k->lengthscale[d];
+    p.inv_ls2[d] = 1.0 / (k->lengthscale[d] * k->lengthscale[d]);
+    if (k->period[d] > 0.0) {
+      p.periodic = 1;
+      p.per[d] = k->period[d];
+      p.pi_over_per[d] = M_PI / k->period[d];
+    }
+  }
+}
+
+static void fill_lik(Prob& p, const pgp_likelihood_t* l) {
+  p.lik_kind = l ? l->kind : PGP_LIK_GAUSSIAN;
+  p.J = l ? l->J : 0;
+  p.sp_bounds = l ? l->single_precision_bounds : 1;
+  p.noise_std = l ? l->noise_std : 1.0;
+  for (int c = 0; c < PGP_MAX_CUT; ++c) p.cut[c] = l ? l->cutpoints[c] : 0.0;
+}
+
+static Prob make_prob(const pgp_problem_t* pr, const pgp_kernel_t* k, const pgp_likelihood_t* l,
+                      int prec, char* ws) {
+  Prob p;
+  memset(&p, 0, sizeof p);
+  const WsLayout L = ws_layout(pr->N, prec);
+  p.N = (int32_t)pr->N;
+  p.Np = (int32_t)L.Np;
+  p.D = pr->D;
+  p.prec = prec;
+  fill_kernel(p, k);
+  fill_lik(p, l);
+  p.X = pr->X;
+  p.y = pr->y;
+  p.K = (double*)(ws + L.K);
+  p.L = ws + L.L;
+  p.logdiag = (double*)(ws + L.logdiag);
+  p.diagval = (double*)(ws + L.diagval);
+  p.dinv = (double*)(ws + L.dinv);
+  p.vec = (double*)(ws + L.vec);
+  p.red = (double*)(ws + OFF_RED);
+  p.info = (int32_t*)(ws + OFF_INFO);
+  p.out_weight = pr->weight;
+  p.out_prec = pr->precision;
+  p.out_latent = pr->latent;
+  return p;
+}
+
+// Read the descriptor's host-visible fields back from a workspace.
+static int read_prob(const char* ws, Prob& p, hipStream_t s) {
+  HIP_OK(hipMemcpyAsync(&p, ws + OFF_PROB, sizeof(Prob), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return 0;
+}
+
+static Batch single_batch(char* ws, int Np) {
+  Batch b;
+  b.dprobs = (Prob* const*)(ws + OFF_ARR);
+  b.count = 1;
+  b.maxNp = Np;
+  return b;
+}
+
+static void set_self(char* ws, hipStream_t s) {
+  std::vector<Prob*> v(1, (Prob*)(ws + OFF_PROB));
+  write_ptrs((Prob**)(ws + OFF_ARR), v, s);
+}
+
+// ---------------------------------------------------------------------------
+// Newton pieces (batched)
+// ---------------------------------------------------------------------------
+static void factor(const Batch& b, int prec, int vb, hipStream_t s) {
+  launch_form_B(b, (prec << 8) | vb, s);
+  launch_potrf(b, prec, s);
+  launch_dinv(b, prec, s);
+}
+
+static void solve_B(const Batch& b, int prec, hipStream_t s) {
+  launch_trsv(b, prec, 0, V_T, V_X3, s);
+  launch_trsv(b, prec, 1, V_X3, V_U, s);
+}
+
+// One Newton iteration z <- z + (I + W K)^-1 (grad - z) for every member of
+// `all`; members of `fac` re-form and re-factor B first.  ||delta||^2 goes to
+// gather[b].
+static void newton_iter(const Batch& all, const Batch* fac, int prec, int refine, double* gather,
+                        hipStream_t s) {
+  launch_gemv_sym(all, V_Z, V_F, 0, s);
+  launch_lik_newton(all, s);
+  if (fac && fac->count > 0) factor(*fac, prec, 0, s);
+  launch_gemv_sym(all, V_R, V_T, 1, s);
+  solve_B(all, prec, s);
+  launch_vec_op(all, OP_STEP, s);
+  for (int r = 0; r < refine; ++r) {
+    launch_gemv_sym(all, V_DL, V_Q, 2, s);
+    launch_gemv_sym(all, V_Q, V_T, 1, s);
+    solve_B(all, prec, s);
+    launch_vec_op(all, OP_IRCORR, s);
+  }
+  launch_vec_op(all, OP_ZUPD, s);
+  launch_reduce(all, RED_STEP, gather, s);
+}
+
+// Final state at the returned weight: f, W, sum log p, and B factored at W.
+static void laplace_finalize(const Batch& all, const Batch* fac, int prec, double* gather,
+                             hipStream_t s) {
+  launch_gemv_sym(all, V_Z, V_F, 0, s);
+  launch_lik_final(all, 0, s);
+  if (fac && fac->count > 0) factor(*fac, prec, 0, s);
+  launch_reduce(all, RED_FINAL, gather, s);
+  launch_vec_op(all, OP_OUT, s);
+}
+
+static void fill_result_final(pgp_fit_result_t* r, const double* g4, int info) {
+  r->sum_log_lik = g4[0];
+  r->half_f_dot_a = 0.5 * g4[1];
+  r->log_det = 2.0 * g4[2];
+  if (g4[3] > 0) r->status |= PGP_INFO_NEGATIVE_W;
+  if (info & PGP_INFO_NOT_PD) r->status |= PGP_INFO_NOT_PD;
+  r->objective = -g4[0] + 0.5 * g4[1] + g4[2];
+  if (r->status & (PGP_INFO_NEGATIVE_W | PGP_INFO_NOT_PD)) r->objective = NAN;
+}
+
+static int default_refine(int prec) { return prec == PGP_PREC_FP64 ? 0 : (prec == PGP_PREC_FP32 ? 1 : 2); }
+
+}  // namespace pgp
+
+using namespace pgp;
+
+extern "C" {
+
+const char* pgp_last_error(void) { return g_err.c_str(); }
+int pgp_version(void) { return 1; }
+
+int pgp_gram(const pgp_kernel_t* kernel, int dtype, const double* X, int64_t N, const double* Z,
+             int64_t M, void* out, int64_t ld, void* stream) {
+  if (!kernel) return fail(-1, "kernel is NULL");
+  if (dtype != 0 && dtype != 1) return fail(-2, "dtype must be 0 (FP64) or 1 (FP32)");
+  if (!X || N <= 0) return fail(-3, "X / N invalid");
+  if (!Z || M <= 0) return fail(-5, "Z / M invalid");
+  if (!out) return fail(-7, "out is NULL");
+  if (ld < N) return fail(-8, "ld (%lld) < N (%lld)", (long long)ld, (long long)N);
+  if (kernel->dim <= 0 || kernel->dim > PGP_MAX_DIM) return fail(-1, "kernel dim out of range");
+  launch_gram_raw(kernel, dtype, X, N, Z, M, out, ld, (hipStream_t)stream);
+  return check_launch();
+}
+
+int pgp_likelihood_eval(const pgp_likelihood_t* lik, const double* f, const double* y, int64_t N,
+                        double* ll, double* d1, double* d2, void* stream) {
+  if (!lik) return fail(-1, "likelihood is NULL");
+  if (!f || !y) return fail(-2, "f / y are NULL");
+  if (N < 0) return fail(-4, "N < 0");
+  launch_lik_raw(lik, f, y, N, ll, d1, d2, (hipStream_t)stream);
+  return check_launch();
+}
+
+int pgp_probit_predictive(const pgp_likelihood_t* lik, const double* mean, const double* var,
+                          int64_t M, double* out, void* stream) {
+  if (!lik) return fail(-1, "likelihood is NULL");
+  if (lik->kind == PGP_LIK_GAUSSIAN) return fail(-1, "needs a probit likelihood");
+  if (lik->J < 2 || lik->J + 1 > PGP_MAX_CUT) return fail(-1, "J out of range");
+  if (!mean || !var || !out) return fail(-2, "NULL pointer");
+  if (M < 0) return fail(-4, "M < 0");
+  launch_probit_predictive(lik, mean, var, M, out, (hipStream_t)stream);
+  return check_launch();
+}
+
+int pgp_potrf(int precision, void* A, int64_t N, int64_t ld, double* logdiag, int32_t* info,
+              void* stream) {
+  if (precision < 0 || precision > 2) return fail(-1, "unknown precision %d", precision);
+  if (!A) return fail(-2, "A is NULL");
+  if (N <= 0 || N % TILE) return fail(-3, "N must be a positive multiple of 64");
+  if (ld < N) return fail(-4, "ld < N");
+  if (!info) return fail(-6, "info must be a device int32");
+  launch_potrf_raw(precision, A, N, ld, logdiag, info, (hipStream_t)stream);
+  return check_launch();
+}
+
+int pgp_trsv(int precision, const void* L, int64_t N, int64_t ld, double* dinv, double* b,
+             double* x, int trans, void* stream) {
+  if (precision < 0 || precision > 2) return fail(-1, "unknown precision %d", precision);
+  if (!L || !b || !x || !dinv) return fail(-2, "NULL pointer");
+  if (N <= 0 || N % TILE) return fail(-3, "N must be a positive multiple of 64");
+  if (ld < N) return fail(-4, "ld < N");
+  hipStream_t s = (hipStream_t)stream;
+  launch_dinv_raw(precision, L, N, ld, dinv, s);
+  launch_trsv_raw_dinv(precision, L, N, ld, dinv, b, x, trans, s);
+  return check_launch();
+}
+
+int pgp_trsm(int precision, const void* L, int64_t N, int64_t ld, double* dinv, void* S,
+             int64_t M, int64_t ld_s, void* stream) {
+  if (precision < 0 || precision > 2) return fail(-1, "unknown precision %d", precision);
+  if (!L || !S || !dinv) return fail(-2, "NULL pointer");
+  if (N <= 0 || N % TILE) return fail(-3, "N must be a positive multiple of 64");
+  if (M <= 0 || M % TILE) return fail(-7, "M must be a positive multiple of 64");
+  if (ld < N || ld_s < N) return fail(-4, "ld < N");
+  hipStream_t s = (hipStream_t)stream;
+  launch_dinv_raw(precision, L, N, ld, dinv, s);
+  launch_trsm_raw_dinv(precision, L, N, ld, dinv, S, M, ld_s, 0, s);
+  return check_launch();
+}
+
+int64_t pgp_workspace_size(int64_t N, int32_t D, int32_t precision) {
+  (void)D;
+  if (N <= 0 || precision < 0 || precision > 2) return -1;
+  return ws_layout(N, precision).total;
+}
+
+int64_t pgp_predict_workspace_size(int64_t N, int64_t M, int32_t precision) {
+  if (N <= 0 || M <= 0 || precision < 0 || precision > 2) return -1;
+  return pred_layout(round_up(N, TILE), M, precision).total;
+}
+
+int64_t pgp_grad_workspace_size(int64_t N, int32_t D) {
+  (void)D;
+  if (N <= 0) return -1;
+  // sized for the widest storage so any precision fits
+  return grad_layout(round_up(N, TILE), PGP_PREC_FP64).total;
+}
+
+int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+                    const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
+                    pgp_fit_result_t* result, void* workspace, int64_t workspace_bytes,
+                    void* stream) {
+  int rc = validate(problem, kernel, lik);
+  if (rc) return rc;
+  if (!lik) return fail(-3, "likelihood is NULL");
+  if (!opts) return fail(-4, "options are NULL");
+  if (!result) return fail(-5, "result is NULL");
+  const int prec = opts->precision;
+  if (prec < 0 || prec > 2) return fail(-4, "unknown precision %d", prec);
+  if (!workspace) return fail(-6, "workspace is NULL");
+  const WsLayout L = ws_layout(problem->N, prec);
+  if (workspace_bytes < L.total)
+    return fail(-7, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+                (long long)L.total);
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  const int refine = opts->refine >= 0 ? opts->refine : default_refine(prec);
+  Prob p = make_prob(problem, kernel, lik, prec, ws);
+  write_prob(ws, p, 1, s);
+  set_self(ws, s);
+  const Batch b = single_batch(ws, p.Np);
+  double* gather = (double*)(ws + OFF_GATHER);
+  launch_gram_self(b, s);
+  launch_vec_op(b, OP_ZERO_Z, s);
+  rc = check_launch();
+  if (rc) return rc;
+  memset(result, 0, sizeof *result);
+  int it = 0;
+  double step = INFINITY;
+  const bool gauss = lik->kind == PGP_LIK_GAUSSIAN;
+  while (step > opts->tolerance && it < opts->maxiter) {
+    const bool refactor = !gauss || it == 0;
+    newton_iter(b, refactor ? &b : nullptr, prec, refine, gather, s);
+    rc = check_launch();
+    if (rc) return rc;
+    double h2;
+    HIP_OK(hipMemcpyAsync(&h2, gather, sizeof h2, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    step = sqrt(h2);
+    ++it;
+    if (!(step == step)) break;  // NaN: stop, report through status
+  }
+  result->iterations = it;
+  result->step_norm = step;
+  if (step > opts->tolerance) result->status |= PGP_INFO_MAXITER;
+  laplace_finalize(b, gauss ? nullptr : &b, prec, gather, s);
+  rc = check_launch();
+  if (rc) return rc;
+  double g4[4];
+  int32_t info[4];
+  HIP_OK(hipMemcpyAsync(g4, gather, sizeof g4, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(info, ws + OFF_INFO, sizeof info, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  fill_result_final(result, g4, info[0]);
+  return 0;
+}
+
+int pgp_laplace_set_state(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+                          const pgp_likelihood_t* lik, int32_t precision, const double* weight,
+                          const double* W, pgp_fit_result_t* result, void* workspace,
+                          int64_t workspace_bytes, void* stream) {
+  int rc = validate(problem, kernel, lik);
+  if (rc) return rc;
+  if (!lik) return fail(-3, "likelihood is NULL");
+  if (precision < 0 || precision > 2) return fail(-4, "unknown precision %d", precision);
+  if (!weight) return fail(-5, "weight is NULL");
+  const WsLayout L = ws_layout(problem->N, precision);
+  if (!workspace || workspace_bytes < L.total) return fail(-9, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  Prob p = make_prob(problem, kernel, lik, precision, ws);
+  write_prob(ws, p, 1, s);
+  set_self(ws, s);
+  const Batch b = single_batch(ws, p.Np);
+  launch_gram_self(b, s);
+  // weight -> Z, f = K a, log p at f, then W and S = sqrt(max(W, 0)) from the caller
+  launch_load_state(b, weight, nullptr, s);
+  launch_gemv_sym(b, V_Z, V_F, 0, s);
+  launch_lik_final(b, 0, s);  // W = -d2 log p(K a), S = sqrt(max(W, 0))
+  if (W) launch_load_state(b, nullptr, W, s);  // caller-given precision instead
+  factor(b, precision, 0, s);
+  double* gather = (double*)(ws + OFF_GATHER);
+  launch_reduce(b, RED_FINAL, gather, s);
+  rc = check_launch();
+  if (rc) return rc;
+  double g4[4];
+  int32_t info[4];
+  HIP_OK(hipMemcpyAsync(g4, gather, sizeof g4, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(info, ws + OFF_INFO, sizeof info, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (result) {
+    memset(result, 0, sizeof *result);
+    fill_result_final(result, g4, info[0]);
+  }
+  launch_vec_op(b, OP_OUT, s);
+  return check_launch();
+}
+
+int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M, double* mean,
+                double* var, void* workspace, int64_t workspace_bytes, void* predict_workspace,
+                int64_t predict_workspace_bytes, void* stream) {
+  (void)kernel;
+  if (!Xtest || M <= 0) return fail(-2, "Xtest / M invalid");
+  if (!mean || !var) return fail(-4, "mean / var are NULL");
+  if (!workspace) return fail(-6, "workspace is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  Prob p;
+  int rc = read_prob(ws, p, s);
+  if (rc) return rc;
+  const WsLayout L = ws_layout(p.N, p.prec);
+  if (workspace_bytes < L.total) return fail(-7, "workspace too small");
+  const PredLayout PL = pred_layout(p.Np, M, p.prec);
+  if (!predict_workspace || predict_workspace_bytes < PL.total)
+    return fail(-9, "predict workspace too small: %lld < %lld",
+                (long long)predict_workspace_bytes, (long long)PL.total);
+  char* pw = (char*)predict_workspace;
+  p.Xt = Xtest;
+  p.M = (int32_t)M;
+  p.Mp = (int32_t)PL.Mp;
+  p.S = pw + PL.S;
+  p.mupart = (double*)(pw + PL.mupart);
+  p.pmean = mean;
+  p.pvar = var;
+  write_prob(ws, p, 0, s);
+  const Batch b = single_batch(ws, p.Np);
+  launch_cross_prep(b, p.prec, (int)M, s);
+  launch_trsm(b, p.prec, (int)PL.Mp, 0, s);
+  launch_predict_final(b, p.prec, (int)M, s);
+  return check_launch();
+}
+
+int pgp_laplace_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik, double* grad_out,
+                     void* workspace, int64_t workspace_bytes, void* grad_workspace,
+                     int64_t grad_workspace_bytes, void* stream) {
+  (void)kernel;
+  (void)lik;
+  if (!grad_out) return fail(-3, "grad_out is NULL");
+  if (!workspace) return fail(-4, "workspace is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  Prob p;
+  int rc = read_prob(ws, p, s);
+  if (rc) return rc;
+  if (p.D > grad_maxd()) return fail(-1, "gradient supports D <= %d (got %d)", grad_maxd(), p.D);
+  if (p.lik_kind == PGP_LIK_PROBIT_CLOSED)
+    return fail(-2, "gradient of the closed-form probit variant is not provided");
+  const WsLayout L = ws_layout(p.N, p.prec);
+  if (workspace_bytes < L.total) return fail(-5, "workspace too small");
+  const GradLayout G = grad_layout(p.Np, PGP_PREC_FP64);
+  if (!grad_workspace || grad_workspace_bytes < G.total)
+    return fail(-7, "gradient workspace too small: %lld < %lld", (long long)grad_workspace_bytes,
+                (long long)G.total);
+  char* gw = (char*)grad_workspace;
+  p.C = (double*)(gw + G.C);
+  p.S = gw + G.G2;
+  p.Mp = p.Np;
+  write_prob(ws, p, 0, s);
+  const Batch b = single_batch(ws, p.Np);
+  const int prec = p.prec;
+  launch_lik_final(b, 1, s);  // G, W, S, LL, D3 at the resident f
+  // C = S Linv^T Linv S
+  launch_fill_rhs(b, prec, 1, s);
+  launch_trsm(b, prec, p.Np, 1, s);
+  launch_sym_from_inv(b, prec, 1, s);
+  // Sigma_ii from V = L^-1 (S K)
+  launch_fill_rhs(b, prec, 0, s);
+  launch_trsm(b, prec, p.Np, 0, s);
+  launch_sigma_diag(b, prec, s);
+  // shat = s2 - C K s2, s2 = -1/2 Sigma d3;  X2 <- K shat
+  launch_vec_op(b, OP_GRAD_S2, s);       // X2 = -0.5 X1 .* D3
+  launch_gemv_sym(b, V_X2, V_X4, 0, s);  // X4 = K s2
+  launch_gemv_sym(b, V_X4, V_T, 10, s);  // T = C X4   (mode 10: matrix C)
+  launch_vec_op(b, OP_GRAD_SHAT, s);     // X3 = X2 - T
+  launch_gemv_sym(b, V_X3, V_X2, 0, s);  // X2 = K shat
+  double* lik_out = (double*)(gw + G.out);
+  double* kern_out = lik_out + 64;
+  launch_lik_grad_terms(b, lik_out, s);
+  launch_grad_trace(b, (double*)(gw + G.partials), kern_out, s);
+  rc = check_launch();
+  if (rc) return rc;
+  std::vector<double> hl(1 + PGP_MAX_CUT), hk(grad_npar());
+  HIP_OK(hipMemcpyAsync(hl.data(), lik_out, hl.size() * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(hk.data(), kern_out, hk.size() * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  const int D = p.D, gm = grad_maxd();
+  int o = 0;
+  grad_out[o++] = hk[0];
+  for (int d = 0; d < D; ++d) grad_out[o++] = hk[1 + d];
+  for (int d = 0; d < D; ++d) grad_out[o++] = p.periodic ? hk[1 + gm + d] : 0.0;
+  grad_out[o++] = hl[0];
+  for (int c = 0; c < PGP_MAX_CUT; ++c) grad_out[o++] = hl[1 + c];
+  return 0;
+}
+
+int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+               const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
+               pgp_fit_result_t* result, void* workspace, int64_t workspace_bytes,
+               void* grad_workspace, int64_t grad_workspace_bytes, void* stream) {
+  int rc = validate(problem, kernel, lik);
+  if (rc) return rc;
+  if (!lik) return fail(-3, "likelihood is NULL");
+  if (!opts || !result) return fail(-4, "options / result are NULL");
+  const int prec = opts->precision;
+  if (prec < 0 || prec > 2) return fail(-4, "unknown precision %d", prec);
+  const WsLayout L = ws_layout(problem->N, prec);
+  if (!workspace || workspace_bytes < L.total) return fail(-7, "workspace too small");
+  const GradLayout G = grad_layout(L.Np, PGP_PREC_FP64);
+  if (!grad_workspace || grad_workspace_bytes < G.total)
+    return fail(-9, "gradient workspace too small (needed for objective_VB)");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  char* gw = (char*)grad_workspace;
+  const int refine = opts->refine >= 0 ? opts->refine : default_refine(prec);
+  Prob p = make_prob(problem, kernel, lik, prec, ws);
+  p.C = (double*)(gw + G.C);
+  p.S = gw + G.G2;
+  p.Mp = p.Np;
+  write_prob(ws, p, 1, s);
+  set_self(ws, s);
+  const Batch b = single_batch(ws, p.Np);
+  double* gather = (double*)(ws + OFF_GATHER);
+  launch_gram_self(b, s);
+  factor(b, prec, 1, s);  // A = sigma^2 I + K
+  launch_vec_op(b, OP_ZERO_Z, s);
+  rc = check_launch();
+  if (rc) return rc;
+  memset(result, 0, sizeof *result);
+  int it = 0;
+  double step = INFINITY;
+  while (step > opts->tolerance && it < opts->maxiter) {
+    launch_gemv_sym(b, V_Z, V_F, 0, s);
+    launch_lik_final(b, 0, s);
+    launch_vec_op(b, OP_VB_RHS, s);
+    launch_copy_slot(b, V_X1, V_T, s);
+    solve_B(b, prec, s);
+    for (int r = 0; r < refine; ++r) {
+      launch_gemv_sym(b, V_U, V_Q, 3, s);
+      launch_copy_slot(b, V_Q, V_T, s);
+      launch_trsv(b, prec, 0, V_T, V_X3, s);
+      launch_trsv(b, prec, 1, V_X3, V_X4, s);
+      launch_vec_op(b, OP_VB_IRCORR, s);
+    }
+    launch_vec_op(b, OP_VB_STEP, s);
+    launch_vec_op(b, OP_ZUPD, s);
+    launch_reduce(b, RED_STEP, gather, s);
+    rc = check_launch();
+    if (rc) return rc;
+    double h2;
+    HIP_OK(hipMemcpyAsync(&h2, gather, sizeof h2, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    step = sqrt(h2);
+    ++it;
+    if (!(step == step)) break;
+  }
+  result->iterations = it;
+  result->step_norm = step;
+  if (step > opts->tolerance) result->status |= PGP_INFO_MAXITER;
+  // objective_VB pieces with the factor of A still resident
+  launch_gemv_sym(b, V_Z, V_F, 0, s);
+  launch_lik_final(b, 0, s);
+  launch_reduce(b, RED_FINAL, gather, s);
+  launch_fill_rhs(b, prec, 1, s);
+  launch_trsm(b, prec, p.Np, 1, s);
+  launch_sym_from_inv(b, prec, 0, s);
+  double* tr_out = (double*)(gw + G.out);
+  launch_vb_trace(b, tr_out, s);
+  // precision = 1/sigma^2 and the B-form factor predict uses
+  launch_vec_op(b, OP_VB_PREC, s);
+  factor(b, prec, 0, s);
+  launch_vec_op(b, OP_OUT, s);
+  rc = check_launch();
+  if (rc) return rc;
+  double g4[4], tr[2];
+  int32_t info[4];
+  HIP_OK(hipMemcpyAsync(g4, gather, sizeof g4, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(tr, tr_out, sizeof tr, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(info, ws + OFF_INFO, sizeof info, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  const double sigma = lik->noise_std;
+  const double N = (double)problem->N;
+  result->sum_log_lik = g4[0];
+  result->half_f_dot_a = 0.5 * g4[1];
+  result->log_det = 2.0 * g4[2];  // log |sigma^2 I + K|
+  if (info[0] & PGP_INFO_NOT_PD) result->status |= PGP_INFO_NOT_PD;
+  // objective_VB (probit/implicit/VB.py:33-39)
+  result->objective = 0.5 * tr[1] + 0.5 * sigma * sigma * tr[0] + 0.5 * g4[1] -
+                      N * log(sigma) + g4[2] - 0.5 * N - g4[0];
+  return 0;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Continuous batching: many fits share every launch of a Newton iteration.
+//
+// A request joins at the next step() after submit(); each step runs one
+// Newton iteration for every request in flight (one launch per blocked
+// Cholesky / solve step for all of them together), then finalises and
+// predicts the requests that converged, which leave the batch.  Every
+// member's arithmetic is independent of the other members (per-member
+// tiles, fixed-order per-member reductions), so a request's result is the
+// same alone and in any mix.
+// ===========================================================================
+namespace pgp {
+
+constexpr int BATCH_ARRAYS = 6;
+
+struct BItem {
+  pgp_batch_item_t* item;
+  int64_t tag;
+  char* ws;
+  Prob p;
+  int phase;  // 0 = new, 1 = Newton
+  int iter;
+  double step;
+  bool gauss;
+};
+
+}  // namespace pgp
+
+struct pgp_batcher {
+  hipStream_t s;
+  pgp_fit_options_t opts;
+  int refine;
+  int cap;
+  Prob** arr[pgp::BATCH_ARRAYS];
+  double* gather;
+  std::vector<double> hg;
+  std::vector<pgp::BItem> items;
+  std::vector<pgp::BItem> pending;
+};
+
+namespace pgp {
+
+static int64_t batcher_table_bytes(int cap) {
+  return align256((int64_t)BATCH_ARRAYS * cap * 8) + align256((int64_t)cap * 8 * 8);
+}
+
+static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v) {
+  std::vector<Prob*> ptrs;
+  ptrs.reserve(v.size());
+  int maxNp = 0;
+  for (BItem* b : v) {
+    ptrs.push_back((Prob*)(b->ws + OFF_PROB));
+    maxNp = std::max(maxNp, (int)b->p.Np);
+  }
+  write_ptrs(bt->arr[which], ptrs, bt->s);
+  Batch b;
+  b.dprobs = bt->arr[which];
+  b.count = (int)v.size();
+  b.maxNp = maxNp;
+  return b;
+}
+
+}  // namespace pgp
+
+extern "C" {
+
+int64_t pgp_batcher_table_size(int32_t capacity) {
+  if (capacity <= 0) return -1;
+  return batcher_table_bytes(capacity);
+}
+
+int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* table,
+                       int64_t table_bytes, void* stream, pgp_batcher_t** out) {
+  if (capacity <= 0) return fail(-1, "capacity must be positive");
+  if (!opts) return fail(-2, "options are NULL");
+  if (opts->precision < 0 || opts->precision > 2) return fail(-2, "unknown precision");
+  if (!table || table_bytes < batcher_table_bytes(capacity))
+    return fail(-3, "batcher table too small");
+  if (!out) return fail(-6, "out is NULL");
+  pgp_batcher* b = new pgp_batcher();
+  b->s = (hipStream_t)stream;
+  b->opts = *opts;
+  b->refine = opts->refine >= 0 ? opts->refine : default_refine(opts->precision);
+  b->cap = capacity;
+  char* t = (char*)table;
+  for (int i = 0; i < BATCH_ARRAYS; ++i) b->arr[i] = (Prob**)(t + (int64_t)i * capacity * 8);
+  b->gather = (double*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8));
+  b->hg.resize((size_t)capacity * 8);
+  *out = b;
+  return 0;
+}
+
+int pgp_batcher_submit(pgp_batcher_t* bt, pgp_batch_item_t* item, int64_t tag) {
+  if (!bt) return fail(-1, "batcher is NULL");
+  if (!item) return fail(-2, "item is NULL");
+  int rc = validate(&item->problem, &item->kernel, &item->lik);
+  if (rc) return rc;
+  const int prec = bt->opts.precision;
+  const WsLayout L = ws_layout(item->problem.N, prec);
+  if (!item->workspace || item->workspace_bytes < L.total)
+    return fail(-2, "item workspace too small: %lld < %lld", (long long)item->workspace_bytes,
+                (long long)L.total);
+  if (item->Xtest) {
+    if (item->M <= 0 || !item->mean || !item->var) return fail(-2, "predict outputs invalid");
+    const PredLayout PL = pred_layout(L.Np, item->M, prec);
+    if (!item->predict_workspace || item->predict_workspace_bytes < PL.total)
+      return fail(-2, "item predict workspace too small");
+  }
+  BItem b;
+  b.item = item;
+  b.tag = tag;
+  b.ws = (char*)item->workspace;
+  b.p = make_prob(&item->problem, &item->kernel, &item->lik, prec, b.ws);
+  b.phase = 0;
+  b.iter = 0;
+  b.step = INFINITY;
+  b.gauss = item->lik.kind == PGP_LIK_GAUSSIAN;
+  memset(&item->result, 0, sizeof item->result);
+  bt->pending.push_back(b);
+  return 0;
+}
+
+int32_t pgp_batcher_in_flight(pgp_batcher_t* bt) {
+  return bt ? (int32_t)(bt->items.size() + bt->pending.size()) : 0;
+}
+
+int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, int32_t* n_done) {
+  if (!bt) return fail(-1, "batcher is NULL");
+  if (!n_done) return fail(-4, "n_done is NULL");
+  *n_done = 0;
+  hipStream_t s = bt->s;
+  const int prec = bt->opts.precision;
+  // admit pending requests up to capacity
+  std::vector<BItem*> fresh;
+  size_t take = 0;
+  while (take < bt->pending.size() && (int)bt->items.size() < bt->cap) {
+    bt->items.push_back(bt->pending[take]);
+    ++take;
+  }
+  bt->pending.erase(bt->pending.begin(), bt->pending.begin() + take);
+  for (BItem& b : bt->items)
+    if (b.phase == 0) {
+      write_prob(b.ws, b.p, 1, s);
+      fresh.push_back(&b);
+    }
+  if (!fresh.empty()) {
+    const Batch bn = make_batch(bt, 0, fresh);
+    launch_gram_self(bn, s);
+    launch_vec_op(bn, OP_ZERO_Z, s);
+    for (BItem* b : fresh) b->phase = 1;
+  }
+  std::vector<BItem*> active, fac;
+  for (BItem& b : bt->items) {
+    active.push_back(&b);
+    if (!b.gauss || b.iter == 0) fac.push_back(&b);
+  }
+  if (active.empty()) return 0;
+  const Batch ba = make_batch(bt, 1, active);
+  const Batch bf = make_batch(bt, 2, fac);
+  newton_iter(ba, &bf, prec, bt->refine, bt->gather, s);
+  int rc = check_launch();
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(bt->hg.data(), bt->gather, active.size() * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  std::vector<BItem*> conv, conv_fac, pred;
+  for (size_t i = 0; i < active.size(); ++i) {
+    BItem* b = active[i];
+    b->step = sqrt(bt->hg[i]);
+    b->iter++;
+    if (b->step <= bt->opts.tolerance || b->iter >= bt->opts.maxiter || !(b->step == b->step)) {
+      conv.push_back(b);
+      if (!b->gauss) conv_fac.push_back(b);
+    }
+  }
+  if (conv.empty()) return 0;
+  const Batch bc = make_batch(bt, 3, conv);
+  const Batch bcf = make_batch(bt, 4, conv_fac);
+  laplace_finalize(bc, &bcf, prec, bt->gather, s);
+  int maxM = 0;
+  for (BItem* b : conv) {
+    pgp_batch_item_t* it = b->item;
+    if (!it->Xtest) continue;
+    const PredLayout PL = pred_layout(b->p.Np, it->M, prec);
+    char* pw = (char*)it->predict_workspace;
+    b->p.Xt = it->Xtest;
+    b->p.M = (int32_t)it->M;
+    b->p.Mp = (int32_t)PL.Mp;
+    b->p.S = pw + PL.S;
+    b->p.mupart = (double*)(pw + PL.mupart);
+    b->p.pmean = it->mean;
+    b->p.pvar = it->var;
+    write_prob(b->ws, b->p, 0, s);
+    pred.push_back(b);
+    maxM = std::max(maxM, (int)it->M);
+  }
+  if (!pred.empty()) {
+    const Batch bp = make_batch(bt, 5, pred);
+    const int maxMp = (int)round_up(maxM, TILE);
+    launch_cross_prep(bp, prec, maxM, s);
+    launch_trsm(bp, prec, maxMp, 0, s);
+    launch_predict_final(bp, prec, maxM, s);
+  }
+  rc = check_launch();
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(bt->hg.data(), bt->gather, conv.size() * 8 * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < conv.size(); ++i) {
+    BItem* b = conv[i];
+    pgp_fit_result_t* r = &b->item->result;
+    memset(r, 0, sizeof *r);
+    r->iterations = b->iter;
+    r->step_norm = b->step;
+    if (b->step > bt->opts.tolerance) r->status |= PGP_INFO_MAXITER;
+    fill_result_final(r, &bt->hg[8 * i], (int)bt->hg[8 * i + 4]);
+    if (done_tags && *n_done < max_done) done_tags[*n_done] = b->tag;
+    ++*n_done;
+  }
+  // drop finished members (stable order)
+  std::vector<BItem> keep;
+  keep.reserve(bt->items.size());
+  for (BItem& b : bt->items) {
+    bool done = false;
+    for (BItem* c : conv)
+      if (c == &b) done = true;
+    if (!done) keep.push_back(b);
+  }
+  bt->items.swap(keep);
+  return 0;
+}
+
+int pgp_batcher_destroy(pgp_batcher_t* bt) {
+  delete bt;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// Internal declarations shared by the HIP translation units of libprobit_amd.
+//
+// Data model: one GP fit is a "problem".  Its device-side descriptor (Prob)
+// lives at the start of the problem's caller-owned workspace and holds the
+// kernel/likelihood parameters plus pointers into the same workspace:
+//   K    : Np x Np FP64 Gram matrix, column-major, ld = Np (zero padding)
+//   L    : Np x Np factor storage (double, or float in mixed modes), lower
+//   dinv : Np/64 FP64 inverses of the 64 x 64 diagonal blocks of L
+//   vec  : NVEC vectors of Np doubles (weight, latent, derivatives, ...)
+// Batched kernels take a device array of descriptor pointers
+// (grid.y or grid.z = batch member), so a single fit is a batch of one.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/probit_amd.h"
+
+namespace pgp {
+
+constexpr int TILE = 64;  // matrix tile / Cholesky panel width
+constexpr int NTHREADS = 256;
+
+// named vector slots in a problem's workspace
+enum Vec {
+  V_Z = 0,  // weight a (Newton / FPI iterate)
+  V_F,      // latent f = K a
+  V_G,      // d1 = d log p / d f
+  V_W,      // W = -d2 log p / d f^2 (unclamped: the returned precision)
+  V_S,      // sqrt(max(W, 0))
+  V_R,      // Newton residual r = d1 - a
+  V_T,      // solve right-hand side / scratch
+  V_U,      // solve output
+  V_DL,     // Newton step delta
+  V_Q,      // refinement residual
+  V_LL,     // per-point log-likelihood
+  V_D3,     // third derivative (gradient path)
+  V_X1,     // scratch
+  V_X2,     // scratch
+  V_X3,     // scratch
+  V_X4,     // scratch
+  NVEC
+};
+
+// reduction slots (doubles) per problem
+enum Red {
+  R_STEP2 = 0,  // ||delta||^2
+  R_SUMLL,      // sum log p
+  R_FA,         // f . a
+  R_LOGDET,     // sum log L_ii (FP64)
+  R_NEGW,       // count of W < 0
+  NRED = 16
+};
+
+struct Prob {
+  int32_t N, Np, D, prec;
+  int32_t kern_kind, periodic, lik_kind, J;
+  int32_t sp_bounds, M, Mp, pad0;
+  double variance, noise_std;
+  double ls[PGP_MAX_DIM];           // lengthscales
+  double inv_ls2[PGP_MAX_DIM];      // 1 / l^2
+  double pi_over_per[PGP_MAX_DIM];  // pi / period (0 if not periodic)
+  double per[PGP_MAX_DIM];          // periods (0 if not periodic)
+  double cut[PGP_MAX_CUT];
+  const double* X;
+  const double* y;
+  double* K;
+  void* L;
+  double* logdiag;
+  double* diagval;
+  double* dinv;
+  double* vec;    // NVEC * Np
+  double* red;    // NRED
+  int32_t* info;  // [4]: [0] PGP_INFO bits, [1] count W < 0
+  double* out_weight;
+  double* out_prec;
+  double* out_latent;
+  // predict
+  const double* Xt;  // M x D row-major
+  void* S;           // Np x Mp factor-precision right-hand sides
+  double* mupart;    // (Np/64) x Mp partial means
+  double* pmean;
+  double* pvar;
+  // gradient / VB objective
+  double* C;  // Np x Np FP64 (C = W^1/2 B^-1 W^1/2, or (sigma^2 I + K)^-1)
+};
+
+__host__ __device__ inline double* vslot(const Prob& p, int v) {
+  return p.vec + (size_t)v * (size_t)p.Np;
+}
+
+struct Batch {
+  Prob* const* dprobs;  // device array of device pointers to descriptors
+  int count;            // batch members in this launch
+  int maxNp;            // max padded size among members
+};
+
+// gram.hip
+void launch_gram_self(const Batch& b, hipStream_t s);
+void launch_form_B(const Batch& b, int prec_vb, hipStream_t s);  // (prec << 8) | vb
+void launch_cross_prep(const Batch& b, int prec, int maxM, hipStream_t s);
+int launch_gram_raw(const pgp_kernel_t* k, int dtype, const double* X, int64_t N,
+                    const double* Z, int64_t M, void* out, int64_t ld, hipStream_t s);
+
+// lik.hip
+void launch_lik_newton(const Batch& b, hipStream_t s);
+void launch_lik_final(const Batch& b, int order3, hipStream_t s);
+int launch_lik_raw(const pgp_likelihood_t* lik, const double* f, const double* y, int64_t N,
+                   double* ll, double* d1, double* d2, hipStream_t s);
+void launch_lik_grad_terms(const Batch& b, double* out, hipStream_t s);
+int launch_probit_predictive(const pgp_likelihood_t* lik, const double* mean, const double* var,
+                             int64_t M, double* out, hipStream_t s);
+
+// chol.hip
+void launch_potrf(const Batch& b, int prec, hipStream_t s);
+void launch_dinv(const Batch& b, int prec, hipStream_t s);
+int launch_potrf_raw(int prec, void* A, int64_t N, int64_t ld, double* logdiag, int32_t* info,
+                     hipStream_t s);
+int launch_dinv_raw(int prec, const void* A, int64_t N, int64_t ld, double* dinv, hipStream_t s);
+
+// solve.hip
+void launch_trsv(const Batch& b, int prec, int trans, int in, int out, hipStream_t s);
+void launch_trsm(const Batch& b, int prec, int maxMp, int lower_rhs, hipStream_t s);
+int launch_trsv_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const double* dinv,
+                         double* b, double* x, int trans, hipStream_t s);
+int launch_trsm_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const double* dinv,
+                         void* S, int64_t M, int64_t lds, int lower_rhs, hipStream_t s);
+
+// vec.hip
+enum VecOp {
+  OP_STEP = 0,
+  OP_IRCORR = 1,
+  OP_ZUPD = 2,
+  OP_ZERO_Z = 3,
+  OP_OUT = 4,
+  OP_VB_RHS = 5,
+  OP_VB_STEP = 6,
+  OP_VB_IRCORR = 7,
+  OP_VB_PREC = 8,
+  OP_GRAD_S2 = 9,    // X2 = -0.5 X1 .* D3
+  OP_GRAD_SHAT = 10, // X3 = X2 - T
+};
+enum RedWhat { RED_STEP = 0, RED_FINAL = 1 };
+void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s);
+void launch_vec_op(const Batch& b, int op, hipStream_t s);
+void launch_copy_slot(const Batch& b, int src, int dst, hipStream_t s);
+void launch_reduce(const Batch& b, int what, double* gather, hipStream_t s);
+void launch_predict_final(const Batch& b, int prec, int maxM, hipStream_t s);
+// weight -> Z (if non-NULL), W -> W and S = sqrt(max(W, 0)) (if non-NULL); zero padding
+void launch_load_state(const Batch& b, const double* weight, const double* W, hipStream_t s);
+
+}  // namespace pgp

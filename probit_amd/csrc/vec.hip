@@ -1,0 +1,261 @@
+// FP64 vector work of the Newton / fixed-point loops and the predict epilogue.
+//
+//  * symmetric GEMV y_i = sum_j K_ji x_j: one wave per output reads one
+//    contiguous column of K (coalesced, deterministic wave reduction) and
+//    applies a fused epilogue (f = K a, S .* K r, refinement residuals).
+//    Replaces `K @ weight` (probit/implicit/Laplace.py:8,22,
+//    probit/approximators.py:273,338) and the matvecs inside the reference's
+//    dense Jacobian (probit/implicit/solvers.py:24).
+//  * elementwise Newton updates and fixed-order per-problem reductions
+//    (step norm for the jaxopt stopping rule, sum log p, f.a, log|B|).
+//  * predict epilogue: mean_j = sum of per-tile partials, var_j = Kss -
+//    ||V_:,j||^2 (probit/approximators.py:176-179).
+#include "pgp_internal.hpp"
+
+namespace pgp {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// mode 0: out = Kx
+// mode 1: out = S .* Kx
+// mode 2: out = R - DL - S^2 .* Kx     (Laplace residual of (I + W K) d = r, x = DL)
+// mode 3: out = X1 - sigma^2 x - Kx    (VB residual of (sigma^2 I + K) w = rhs, rhs in X1)
+// mode 4: out = x + Kx... unused
+__global__ __launch_bounds__(NTHREADS) void k_gemv_sym(Prob* const* probs, int xs, int os,
+                                                       int mode) {
+  const Prob& P = *probs[blockIdx.y];
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= P.Np) return;
+  const int lane = threadIdx.x & 63;
+  const double* col = (mode >= 10 ? P.C : P.K) + (size_t)i * P.Np;
+  mode %= 10;
+  const double* x = vslot(P, xs);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int j = lane;
+  for (; j + 192 < P.Np; j += 256) {
+    a0 += col[j] * x[j];
+    a1 += col[j + 64] * x[j + 64];
+    a2 += col[j + 128] * x[j + 128];
+    a3 += col[j + 192] * x[j + 192];
+  }
+  for (; j < P.Np; j += 64) a0 += col[j] * x[j];
+  const double sum = wave_sum((a0 + a1) + (a2 + a3));
+  if (lane != 0) return;
+  double v;
+  switch (mode) {
+    case 0:
+      v = sum;
+      break;
+    case 1:
+      v = vslot(P, V_S)[i] * sum;
+      break;
+    case 2: {
+      const double si = vslot(P, V_S)[i];
+      v = vslot(P, V_R)[i] - x[i] - si * si * sum;
+      break;
+    }
+    default: {
+      const double s2 = (i < P.N) ? P.noise_std * P.noise_std : 1.0;
+      v = vslot(P, V_X1)[i] - s2 * x[i] - sum;
+      break;
+    }
+  }
+  vslot(P, os)[i] = v;
+}
+
+void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s) {
+  dim3 g((b.maxNp + 3) / 4, b.count);
+  hipLaunchKernelGGL(k_gemv_sym, g, dim3(NTHREADS), 0, s, b.dprobs, x_slot, out_slot, mode);
+}
+
+
+
+__global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, int op) {
+  const Prob& P = *probs[blockIdx.y];
+  const int i = blockIdx.x * NTHREADS + threadIdx.x;
+  if (i >= P.Np) return;
+  switch (op) {
+    case OP_STEP:
+      vslot(P, V_DL)[i] = vslot(P, V_R)[i] - vslot(P, V_S)[i] * vslot(P, V_U)[i];
+      break;
+    case OP_IRCORR:
+      vslot(P, V_DL)[i] += vslot(P, V_Q)[i] - vslot(P, V_S)[i] * vslot(P, V_U)[i];
+      break;
+    case OP_ZUPD:
+      vslot(P, V_Z)[i] += vslot(P, V_DL)[i];
+      break;
+    case OP_ZERO_Z:
+      vslot(P, V_Z)[i] = 0.0;
+      break;
+    case OP_OUT:
+      if (i < P.N) {
+        if (P.out_weight) P.out_weight[i] = vslot(P, V_Z)[i];
+        if (P.out_prec) P.out_prec[i] = vslot(P, V_W)[i];
+        if (P.out_latent) P.out_latent[i] = vslot(P, V_F)[i];
+      }
+      break;
+    case OP_VB_RHS:
+      vslot(P, V_X1)[i] = vslot(P, V_F)[i] + P.noise_std * vslot(P, V_G)[i];
+      break;
+    case OP_VB_STEP:
+      vslot(P, V_DL)[i] = vslot(P, V_U)[i] - vslot(P, V_Z)[i];
+      break;
+    case OP_VB_IRCORR:
+      vslot(P, V_U)[i] += vslot(P, V_X4)[i];
+      break;
+    case OP_GRAD_S2:
+      vslot(P, V_X2)[i] = -0.5 * vslot(P, V_X1)[i] * vslot(P, V_D3)[i];
+      break;
+    case OP_GRAD_SHAT:
+      vslot(P, V_X3)[i] = vslot(P, V_X2)[i] - vslot(P, V_T)[i];
+      break;
+    case OP_VB_PREC: {
+      const double w = (i < P.N) ? 1.0 / (P.noise_std * P.noise_std) : 0.0;
+      vslot(P, V_W)[i] = w;
+      vslot(P, V_S)[i] = sqrt(w);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+void launch_vec_op(const Batch& b, int op, hipStream_t s) {
+  dim3 g((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
+  hipLaunchKernelGGL(k_vec, g, dim3(NTHREADS), 0, s, b.dprobs, op);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_load_state(Prob* const* probs, const double* weight,
+                                                         const double* W) {
+  const Prob& P = *probs[blockIdx.y];
+  const int i = blockIdx.x * NTHREADS + threadIdx.x;
+  if (i >= P.Np) return;
+  if (weight) vslot(P, V_Z)[i] = (i < P.N) ? weight[i] : 0.0;
+  if (W) {
+    const double w = (i < P.N) ? W[i] : 0.0;
+    vslot(P, V_W)[i] = w;
+    vslot(P, V_S)[i] = sqrt(fmax(w, 0.0));
+  }
+}
+
+void launch_load_state(const Batch& b, const double* weight, const double* W, hipStream_t s) {
+  dim3 g((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
+  hipLaunchKernelGGL(k_load_state, g, dim3(NTHREADS), 0, s, b.dprobs, weight, W);
+}
+
+// Copy a vector slot between problems' slots (same problem): dst = src.
+__global__ __launch_bounds__(NTHREADS) void k_copy_slot(Prob* const* probs, int src, int dst) {
+  const Prob& P = *probs[blockIdx.y];
+  const int i = blockIdx.x * NTHREADS + threadIdx.x;
+  if (i >= P.Np) return;
+  vslot(P, dst)[i] = vslot(P, src)[i];
+}
+
+void launch_copy_slot(const Batch& b, int src, int dst, hipStream_t s) {
+  dim3 g((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
+  hipLaunchKernelGGL(k_copy_slot, g, dim3(NTHREADS), 0, s, b.dprobs, src, dst);
+}
+
+// ---- fixed-order block reduction ----
+__device__ double block_sum(double v) {
+  __shared__ double ws[NTHREADS / 64];
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+
+
+__global__ __launch_bounds__(NTHREADS) void k_reduce(Prob* const* probs, int what,
+                                                     double* gather) {
+  const Prob& P = *probs[blockIdx.x];
+  if (what == RED_STEP) {
+    const double* d = vslot(P, V_DL);
+    double a = 0.0;
+    for (int i = threadIdx.x; i < P.Np; i += NTHREADS) a += d[i] * d[i];
+    const double s = block_sum(a);
+    if (threadIdx.x == 0) {
+      P.red[R_STEP2] = s;
+      if (gather) gather[blockIdx.x] = s;
+    }
+  } else {
+    const double* ll = vslot(P, V_LL);
+    const double* f = vslot(P, V_F);
+    const double* z = vslot(P, V_Z);
+    const double* w = vslot(P, V_W);
+    double a = 0.0, b = 0.0, c = 0.0, n = 0.0;
+    for (int i = threadIdx.x; i < P.Np; i += NTHREADS) {
+      a += ll[i];
+      b += f[i] * z[i];
+      c += P.logdiag[i];
+      n += (i < P.N && w[i] < 0.0) ? 1.0 : 0.0;
+    }
+    const double sa = block_sum(a);
+    const double sb = block_sum(b);
+    const double sc = block_sum(c);
+    const double sn = block_sum(n);
+    if (threadIdx.x == 0) {
+      P.red[R_SUMLL] = sa;
+      P.red[R_FA] = sb;
+      P.red[R_LOGDET] = sc;
+      P.red[R_NEGW] = sn;
+      if (gather) {
+        gather[8 * blockIdx.x + 0] = sa;
+        gather[8 * blockIdx.x + 1] = sb;
+        gather[8 * blockIdx.x + 2] = sc;
+        gather[8 * blockIdx.x + 3] = sn;
+        gather[8 * blockIdx.x + 4] = (double)P.info[0];
+      }
+    }
+  }
+}
+
+void launch_reduce(const Batch& b, int what, double* gather, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce, dim3(b.count), dim3(NTHREADS), 0, s, b.dprobs, what, gather);
+}
+
+// ---- predict epilogue: one wave per test point ----
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_predict_final(Prob* const* probs) {
+  const Prob& P = *probs[blockIdx.y];
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= P.M) return;
+  const int lane = threadIdx.x & 63;
+  const T* S = (const T*)P.S + (size_t)j * P.Np;
+  double q0 = 0.0, q1 = 0.0;
+  int i = lane;
+  for (; i + 64 < P.Np; i += 128) {
+    const double v0 = (double)S[i], v1 = (double)S[i + 64];
+    q0 += v0 * v0;
+    q1 += v1 * v1;
+  }
+  for (; i < P.Np; i += 64) {
+    const double v = (double)S[i];
+    q0 += v * v;
+  }
+  const double q = wave_sum(q0 + q1);
+  double mu = 0.0;
+  const int nrt = P.Np / TILE;
+  for (int rt = lane; rt < nrt; rt += 64) mu += P.mupart[(size_t)rt * P.Mp + j];
+  mu = wave_sum(mu);
+  if (lane == 0) {
+    P.pmean[j] = mu;
+    P.pvar[j] = P.variance - q;
+  }
+}
+
+void launch_predict_final(const Batch& b, int prec, int maxM, hipStream_t s) {
+  dim3 g((maxM + 3) / 4, b.count);
+  if (prec == PGP_PREC_FP64)
+    hipLaunchKernelGGL(k_predict_final<double>, g, dim3(NTHREADS), 0, s, b.dprobs);
+  else
+    hipLaunchKernelGGL(k_predict_final<float>, g, dim3(NTHREADS), 0, s, b.dprobs);
+}
+
+}  // namespace pgp

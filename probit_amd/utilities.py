@@ -1,0 +1,197 @@
+"""Likelihoods and helpers with the reference's names and signatures
+(probit/utilities.py).
+
+The likelihood functions are the identity tokens the approximators dispatch
+on: passing ``log_gaussian_likelihood`` or ``log_probit_likelihood`` (and
+optionally the closed-form ``grad_/hessian_log_probit_likelihood`` pair)
+selects the matching fused HIP kernel inside the Newton step.  Called
+directly, they evaluate elementwise on the GPU through the same kernel
+(``pgp_likelihood_eval``), with the reference's argument convention
+``f(f, y, likelihood_parameters)``.
+"""
+
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+import torch
+
+from . import _lib
+
+__all__ = [
+    "BOUNDS",
+    "h",
+    "log_gaussian_likelihood",
+    "log_probit_likelihood",
+    "grad_log_probit_likelihood",
+    "hessian_log_probit_likelihood",
+    "probit_predictive_distributions",
+    "check_data",
+    "check_cutpoints",
+    "CutpointValueError",
+    "InvalidKernel",
+]
+
+BOUNDS = {"single": [1.3, 1.8, 2.3], "double": [2.3, 3.6, 4.8]}  # probit/utilities.py:15
+
+
+def h(x):
+    """Polynomial part of the normal log-survival tail series
+    (probit/utilities.py:37-44): -x^-2 + 5/2 x^-4 - 37/3 x^-6.  A host
+    helper; the device copy lives in the closed-form likelihood kernel."""
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return -1.0 * x**-2 + 5.0 / 2.0 * x**-4 - 37.0 / 3.0 * x**-6
+
+
+# --------------------------------------------------------------------------
+# likelihood parameter parsing
+# --------------------------------------------------------------------------
+def _scalar(v) -> float:
+    if isinstance(v, torch.Tensor):
+        return float(v.detach().reshape(()))
+    return float(np.asarray(v, dtype=np.float64).reshape(()))
+
+
+def likelihood_struct(kind: int, likelihood_parameters, single_precision: bool = True
+                      ) -> _lib.LikelihoodT:
+    """Pack likelihood parameters ((noise_std,) or (noise_std, cutpoints))."""
+    lp = likelihood_parameters
+    if not isinstance(lp, (tuple, list)):
+        lp = (lp,)
+    s = _lib.LikelihoodT()
+    s.kind = kind
+    s.noise_std = _scalar(lp[0])
+    s.single_precision_bounds = 1 if single_precision else 0
+    if kind != _lib.LIK_GAUSSIAN:
+        if len(lp) < 2:
+            raise ValueError("probit likelihood parameters are (noise_std, cutpoints)")
+        cut = lp[1]
+        cut = cut.detach().cpu().numpy() if isinstance(cut, torch.Tensor) else np.asarray(cut)
+        cut = np.asarray(cut, dtype=np.float64).reshape(-1)
+        J = cut.shape[0] - 1
+        if J < 2 or J + 1 > _lib.MAX_CUT:
+            raise ValueError(f"need 3..{_lib.MAX_CUT} cutpoints incl. +-inf (got {J + 1})")
+        s.J = J
+        for c in range(J + 1):
+            s.cutpoints[c] = float(cut[c])
+    return s
+
+
+def _eval(kind, f, y, likelihood_parameters, which, single_precision=True):
+    lib = _lib.lib()
+    ft = torch.as_tensor(f, dtype=torch.float64)
+    shape = ft.shape
+    dev = ft.device if ft.is_cuda else torch.device("cuda")
+    fd = ft.reshape(-1).to(dev).contiguous()
+    yd = torch.as_tensor(y, dtype=torch.float64).reshape(-1).to(dev)
+    yd = torch.broadcast_to(yd, fd.shape).contiguous()
+    out = torch.empty_like(fd)
+    ls = likelihood_struct(kind, likelihood_parameters, single_precision)
+    ptrs = [None, None, None]
+    ptrs[which] = out.data_ptr()
+    rc = lib.pgp_likelihood_eval(ls, fd.data_ptr(), yd.data_ptr(), fd.numel(), *ptrs,
+                                 _lib.stream_handle(dev))
+    _lib.check(rc, "pgp_likelihood_eval")
+    return out.reshape(shape)
+
+
+def log_gaussian_likelihood(f, y, likelihood_parameters):
+    """log N(y; f, noise_std^2) (probit/utilities.py:60-61)."""
+    return _eval(_lib.LIK_GAUSSIAN, f, y, likelihood_parameters, 0)
+
+
+def log_probit_likelihood(f, y, likelihood_parameters):
+    """log(Phi(z2) - Phi(z1) + 1e-10) of the class interval (probit/utilities.py:56-57)."""
+    return _eval(_lib.LIK_PROBIT, f, y, likelihood_parameters, 0)
+
+
+def grad_log_probit_likelihood(f, y, likelihood_parameters, single_precision=True):
+    """Closed-form gradient with BOUNDS tails (probit/utilities.py:151-169)."""
+    return _eval(_lib.LIK_PROBIT_CLOSED, f, y, likelihood_parameters, 1, single_precision)
+
+
+def hessian_log_probit_likelihood(f, y, likelihood_parameters, single_precision=True):
+    """Closed-form second derivative with BOUNDS tails (probit/utilities.py:172-192)."""
+    return _eval(_lib.LIK_PROBIT_CLOSED, f, y, likelihood_parameters, 2, single_precision)
+
+
+def probit_predictive_distributions(likelihood_parameters, posterior_mean, posterior_variance):
+    """(N_test, J) ordinal class probabilities (probit/utilities.py:232-249)."""
+    lib = _lib.lib()
+    mean = torch.as_tensor(posterior_mean, dtype=torch.float64)
+    dev = mean.device if mean.is_cuda else torch.device("cuda")
+    mean = mean.reshape(-1).to(dev).contiguous()
+    var = torch.as_tensor(posterior_variance, dtype=torch.float64).reshape(-1).to(dev).contiguous()
+    ls = likelihood_struct(_lib.LIK_PROBIT, likelihood_parameters)
+    out = torch.empty((mean.numel(), ls.J), dtype=torch.float64, device=dev)
+    rc = lib.pgp_probit_predictive(ls, mean.data_ptr(), var.data_ptr(), mean.numel(),
+                                   out.data_ptr(), _lib.stream_handle(dev))
+    _lib.check(rc, "pgp_probit_predictive")
+    return out
+
+
+# --------------------------------------------------------------------------
+# validators (probit/utilities.py:252-361), restated with working padding
+# --------------------------------------------------------------------------
+def check_data(data):
+    """Class labels must be integers (probit/utilities.py:252-260)."""
+    X_train, y_train = data
+    dt = y_train.dtype
+    is_int = (np.issubdtype(np.dtype(str(dt).replace("torch.", "")), np.integer)
+              if not isinstance(dt, np.dtype) else np.issubdtype(dt, np.integer))
+    if not is_int:
+        raise TypeError(f"t must contain only integer values (got {dt})")
+    return X_train, y_train.astype(int) if isinstance(y_train, np.ndarray) else y_train.long()
+
+
+def check_cutpoints(cutpoints, J):
+    """Pad / validate cutpoints to J+1 entries with -inf and +inf ends
+    (probit/utilities.py:263-322)."""
+    warnings.warn("Checking cutpoints...")
+    c = cutpoints.detach().cpu().numpy() if isinstance(cutpoints, torch.Tensor) else cutpoints
+    c = np.asarray(c, dtype=np.float64).reshape(-1)
+    if c.shape[0] == J - 1:
+        c = np.concatenate([[-np.inf], c, [np.inf]])
+    elif c.shape[0] == J:
+        if c[-1] != np.inf:
+            if c[0] != -np.inf:
+                raise ValueError(
+                    "Either the largest cutpoint parameter b_J is not positive infinity, or the "
+                    f"smallest cutpoint parameter b_0 is not negative infinity (got "
+                    f"{[c[0], c[-1]]}, expected {[-np.inf, np.inf]})")
+            c = np.concatenate([c, [np.inf]])
+        else:
+            c = np.concatenate([[-np.inf], c])
+    elif c.shape[0] == J + 1:
+        if c[0] != -np.inf:
+            raise ValueError(
+                f"The smallest cutpoint parameter b_0 must be negative infinity (got {c[0]})")
+        if c[-1] != np.inf:
+            raise ValueError(
+                f"The largest cutpoint parameter b_J must be positive infinity (got {c[-1]})")
+    else:
+        raise ValueError(f"Could not recognise cutpoints shape. (shape was {c.shape})")
+    if not all(c[i] <= c[i + 1] for i in range(J)):
+        raise CutpointValueError(c)
+    return c
+
+
+class CutpointValueError(Exception):
+    """An invalid cutpoint argument (probit/utilities.py:325-345)."""
+
+    def __init__(self, cutpoint):
+        super().__init__(
+            f"The cutpoint list or array must be in ascending order,  {cutpoint} was given.")
+
+
+class InvalidKernel(Exception):
+    """The prior did not return a kernel (probit/utilities.py:348-361)."""
+
+    def __init__(self, kernel):
+        super().__init__(f"{kernel} is not an instance of probit_amd.kernels.Kernel")
+
+
+SQRT2 = math.sqrt(2.0)

@@ -1,0 +1,201 @@
+"""Parity of the drop-in LaplaceGP / VBGP / serving path against the oracle,
+at sizes the oracle finishes in seconds."""
+
+import numpy as np
+import pytest
+
+from oracle import gp_oracle as O
+from tests.helpers import lik_kind, make_prior, rel_err, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _gp(cls_name, X, y, kind, lik, tol=1e-5, **kw):
+    import probit_amd as P
+    cls = getattr(P, cls_name)
+    ll = P.log_gaussian_likelihood if lik == "gaussian" else P.log_probit_likelihood
+    return cls(data=(X, y), prior=make_prior(kind), log_likelihood=ll, tolerance=tol, **kw)
+
+
+@pytest.mark.parametrize("lik", ["gaussian", "probit"])
+@pytest.mark.parametrize("kind", ["EQ", "Matern12"])
+@pytest.mark.parametrize("N,D", [(37, 1), (200, 3)])
+def test_laplace_matches_reference_iterates(lik, kind, N, D):
+    X, y, theta, lp, spec = small_problem(N, N, D, lik, kind)
+    gp = _gp("LaplaceGP", X, y, kind, lik)
+    params = (theta, lp)
+    w, p = gp.approximate_posterior(params)
+    K = O.gram(spec, X)
+    w_ref, it_ref, _ = O.laplace_weight(K, y, lik_kind(lik), lp, tol=1e-5)
+    assert gp.last_result.iterations == it_ref
+    assert rel_err(w.cpu().numpy(), w_ref) < 1e-9
+    p_ref, _ = O.laplace_precision(K, y, lik_kind(lik), lp, w_ref)
+    assert rel_err(p.cpu().numpy(), p_ref) < 1e-9
+    Xt = np.random.default_rng(5).normal(size=(150, D))
+    mean, var = gp.predict(Xt, params, w, p)
+    m_ref, v_ref = O.predict(spec, X, Xt, w_ref, p_ref)
+    assert rel_err(mean.cpu().numpy(), m_ref) < 1e-9
+    assert rel_err(var.cpu().numpy(), v_ref) < 1e-9
+    obj = float(gp.objective()(params))
+    ref = O.objective_LA(spec, X, y, lik_kind(lik), lp, w_ref)
+    assert abs(obj - ref) <= 1e-9 * abs(ref)
+
+
+@pytest.mark.parametrize("lik", ["gaussian", "probit"])
+@pytest.mark.parametrize("kind,period", [("EQ", None), ("Matern12", None), ("EQ", 0.9)])
+def test_laplace_gradient_matches_closed_form(lik, kind, period):
+    import probit_amd as P
+    X, y, theta, lp, spec = small_problem(3, 120, 2, lik, kind, period=period)
+    ll = P.log_gaussian_likelihood if lik == "gaussian" else P.log_probit_likelihood
+    gp = P.LaplaceGP(data=(X, y), prior=make_prior(kind, period), log_likelihood=ll,
+                     tolerance=1e-11)
+    val, grads = gp.value_and_grad()((theta, lp))
+    K = O.gram(spec, X)
+    w = O.laplace_weight_converged(K, y, lik_kind(lik), lp)
+    ref = O.laplace_objective_gradient(spec, X, y, lik_kind(lik), lp, w)
+    assert abs(val - O.objective_LA(spec, X, y, lik_kind(lik), lp, w)) < 1e-9 * abs(val)
+    (g_ls, g_var), g_lik = grads
+    ref_ls = sum(ref[f"lengthscale{d}"] for d in range(2))
+    assert abs(g_ls - ref_ls) <= 1e-7 * max(1.0, abs(ref_ls))
+    assert abs(g_var - ref["variance"]) <= 1e-7 * max(1.0, abs(ref["variance"]))
+    assert abs(g_lik[0] - ref["noise_std"]) <= 1e-7 * max(1.0, abs(ref["noise_std"]))
+    if lik == "probit":
+        assert np.allclose(g_lik[1], ref["cutpoints"], rtol=1e-7, atol=1e-9)
+
+
+def test_periodic_gradient_reaches_period_parameter():
+    """A prior whose period is a parameter gets d/dperiod from the adjoint."""
+    import torch
+
+    import probit_amd as P
+    X, y, theta, lp, spec = small_problem(4, 90, 1, "gaussian", "EQ", period=0.7)
+
+    def prior(th):
+        ls, var, per = th
+        return var * P.EQ().stretch(ls).periodic(per)
+
+    gp = P.LaplaceGP(data=(X, y), prior=prior, log_likelihood=P.log_gaussian_likelihood,
+                     tolerance=1e-11)
+    val, grads = gp.value_and_grad()(((theta[0], theta[1], 0.7), lp))
+    K = O.gram(spec, X)
+    w = O.laplace_weight_converged(K, y, O.GAUSSIAN, lp)
+    ref = O.laplace_objective_gradient(spec, X, y, O.GAUSSIAN, lp, w)
+    assert abs(grads[0][2] - ref["period0"]) <= 1e-7 * max(1.0, abs(ref["period0"]))
+    del torch
+
+
+@pytest.mark.parametrize("lik", ["gaussian", "probit"])
+def test_vb_matches_reference(lik):
+    X, y, theta, lp, spec = small_problem(11, 96, 2, lik, "EQ", noise=1.0 if lik == "gaussian"
+                                          else 0.8)
+    gp = _gp("VBGP", X, y, "EQ", lik, tol=1e-5)
+    params = (theta, lp)
+    w, p = gp.approximate_posterior(params)
+    K = O.gram(spec, X)
+    w_ref, it_ref, _ = O.vb_weight(K, y, lik_kind(lik), lp, tol=1e-5)
+    assert gp.last_result.iterations == it_ref
+    assert rel_err(w.cpu().numpy(), w_ref) < 1e-8
+    assert np.allclose(p.cpu().numpy(), O.vb_precision(lp, 96))
+    obj = float(gp.objective()(params))
+    ref = O.objective_VB(spec, X, y, lik_kind(lik), lp, w_ref)
+    assert abs(obj - ref) <= 1e-8 * abs(ref)
+    Xt = np.random.default_rng(2).normal(size=(64, 2))
+    mean, var = gp.predict(Xt, params, w, p)
+    m_ref, v_ref = O.predict(spec, X, Xt, w_ref, O.vb_precision(lp, 96))
+    assert rel_err(mean.cpu().numpy(), m_ref) < 1e-8
+    assert rel_err(var.cpu().numpy(), v_ref) < 1e-8
+
+
+def test_predict_with_foreign_weight_and_precision():
+    """predict() with arrays that did not come from this GP re-factors from them."""
+    X, y, theta, lp, spec = small_problem(12, 80, 2, "probit", "EQ")
+    gp = _gp("LaplaceGP", X, y, "EQ", "probit")
+    K = O.gram(spec, X)
+    w = O.laplace_weight_converged(K, y, O.PROBIT, lp)
+    p, _ = O.laplace_precision(K, y, O.PROBIT, lp, w)
+    Xt = np.random.default_rng(3).normal(size=(40, 2))
+    mean, var = gp.predict(Xt, (theta, lp), w, p)
+    m_ref, v_ref = O.predict(spec, X, Xt, w, p)
+    assert rel_err(mean.cpu().numpy(), m_ref) < 1e-10
+    assert rel_err(var.cpu().numpy(), v_ref) < 1e-10
+
+
+def test_precision_and_construct():
+    X, y, theta, lp, spec = small_problem(13, 70, 2, "probit", "Matern12")
+    gp = _gp("LaplaceGP", X, y, "Matern12", "probit")
+    K = O.gram(spec, X)
+    w = O.laplace_weight_converged(K, y, O.PROBIT, lp)
+    W, f = gp.precision(w, (theta, lp))
+    W_ref, f_ref = O.laplace_precision(K, y, O.PROBIT, lp, w)
+    assert rel_err(W.cpu().numpy(), W_ref) < 1e-11
+    assert rel_err(f.cpu().numpy(), f_ref) < 1e-11
+    g = gp.construct()((theta, lp), w)
+    g_ref = O.lik_derivs(O.PROBIT, f_ref, y, lp)["d1"]
+    assert rel_err(g.cpu().numpy(), g_ref) < 1e-10
+
+
+@pytest.mark.parametrize("mode,tol", [("fp32", 1e-5), ("bf16", 1e-5)])
+def test_mixed_precision_mean_within_tolerance(mode, tol):
+    """Mixed-precision factors with FP64 residuals and refinement keep the
+    predictive mean within 1e-5 relative of the FP64 oracle."""
+    X, y, theta, lp, spec = small_problem(21, 700, 3, "probit", "EQ", ls=1.0, var=1.0,
+                                          noise=0.5)
+    gp = _gp("LaplaceGP", X, y, "EQ", "probit", precision_mode=mode)
+    params = (theta, lp)
+    w, p = gp.approximate_posterior(params)
+    Xt = np.random.default_rng(4).normal(size=(256, 3))
+    mean, var = gp.predict(Xt, params, w, p)
+    K = O.gram(spec, X)
+    w_ref = O.laplace_weight_converged(K, y, O.PROBIT, lp)
+    p_ref, _ = O.laplace_precision(K, y, O.PROBIT, lp, w_ref)
+    m_ref, v_ref = O.predict(spec, X, Xt, w_ref, p_ref)
+    assert rel_err(mean.cpu().numpy(), m_ref) < tol
+
+
+def test_serving_alone_equals_in_mix():
+    """A request's results are bit-identical alone and in a mixed batch, and
+    match the single-problem entry point and the oracle."""
+    from probit_amd.serving import ServingEngine
+    probs = []
+    for i, (N, D, lik, kind) in enumerate([(150, 2, "probit", "EQ"), (300, 1, "gaussian",
+                                                                      "Matern12"),
+                                           (64, 4, "probit", "Matern12"),
+                                           (500, 3, "gaussian", "EQ")]):
+        X, y, theta, lp, spec = small_problem(30 + i, N, D, lik, kind)
+        Xt = np.random.default_rng(40 + i).normal(size=(100, D))
+        probs.append((X, y, theta, lp, spec, lik, kind, Xt))
+
+    def run(idx, mode="fp64"):
+        eng = ServingEngine(precision=mode, capacity=8, hbm_budget_bytes=1 << 32)
+        for i in idx:
+            X, y, theta, lp, spec, lik, kind, Xt = probs[i]
+            assert eng.submit(i, X, y, kind, theta[0], theta[1], lik, lp[0],
+                              lp[1] if lik == "probit" else None, Xt)
+        out = {}
+        while eng.in_flight:
+            for c in eng.step():
+                out[c.tag] = c
+        eng.close()
+        return out
+
+    alone = run([0])
+    mix = run([3, 1, 0, 2])
+    for k in ("mean", "var", "weight", "precision"):
+        a = getattr(alone[0], k).cpu().numpy()
+        b = getattr(mix[0], k).cpu().numpy()
+        assert np.array_equal(a, b), k
+    for i in range(4):
+        X, y, theta, lp, spec, lik, kind, Xt = probs[i]
+        K = O.gram(spec, X)
+        w_ref, it_ref, _ = O.laplace_weight(K, y, lik_kind(lik), lp, tol=1e-5)
+        p_ref, _ = O.laplace_precision(K, y, lik_kind(lik), lp, w_ref)
+        m_ref, v_ref = O.predict(spec, X, Xt, w_ref, p_ref)
+        c = mix[i]
+        assert c.iterations == it_ref
+        assert rel_err(c.mean.cpu().numpy(), m_ref) < 1e-9
+        assert rel_err(c.var.cpu().numpy(), v_ref) < 1e-9
+    # same in mixed precision: alone == in the mix, bit for bit
+    a = run([2], "bf16")
+    b = run([0, 1, 2, 3], "bf16")
+    assert np.array_equal(a[2].mean.cpu().numpy(), b[2].mean.cpu().numpy())
+    assert np.array_equal(a[2].var.cpu().numpy(), b[2].var.cpu().numpy())
