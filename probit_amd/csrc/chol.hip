@@ -178,7 +178,7 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj) 
       const int64_t col = rj + acc_col(bn);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = ri + acc_row(bm, r);
+        const int64_t row = ri + acc_row<T>(bm, r);
         T* p = &A[(size_t)row + (size_t)col * ld];
         *p = *p - acc[bm][bn][r];
       }

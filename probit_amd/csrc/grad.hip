@@ -70,7 +70,7 @@ __global__ __launch_bounds__(NTHREADS) void k_sym_from_inv(Prob* const* probs, i
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int i = ti * TILE + acc_row(bm, rr);
+        const int i = ti * TILE + acc_row<T>(bm, rr);
         const int j = tj * TILE + acc_col(bn);
         double v = (double)acc[bm][bn][rr];
         if (use_scale) v *= S[i] * S[j];

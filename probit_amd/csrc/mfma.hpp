@@ -10,7 +10,9 @@
 // Lane layout of the 16x16xK MFMA family (lane l = threadIdx.x & 63):
 //   A operand: row (l & 15), k (l >> 4) [x8 consecutive k for bf16 x32]
 //   B operand: col (l & 15), k (l >> 4) [x8 consecutive k for bf16 x32]
-//   D:         row 4*(l >> 4) + r (r = 0..3), col (l & 15)
+//   D (f32, bf16 inputs): row 4*(l >> 4) + r (r = 0..3), col (l & 15)
+//   D (f64 16x16x4):      row (l >> 4) + 4*r  (r = 0..3), col (l & 15)
+// (the f64 row interleave was measured on gfx950 with a lane-layout probe).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -103,10 +105,16 @@ __device__ __forceinline__ void mma64_bf16(const unsigned short* As, int pa,
   }
 }
 
-// Global (row, col) of accumulator element (bm, bn, r) relative to the tile.
+// Row / column of accumulator element (bm, bn, r) relative to the 64x64 tile.
+template <typename T>
 __device__ __forceinline__ int acc_row(int bm, int r) {
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   return 32 * (w >> 1) + 16 * bm + 4 * (l >> 4) + r;
+}
+template <>
+__device__ __forceinline__ int acc_row<double>(int bm, int r) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  return 32 * (w >> 1) + 16 * bm + (l >> 4) + 4 * r;
 }
 __device__ __forceinline__ int acc_col(int bn) {
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;

@@ -173,7 +173,7 @@ __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* 
       for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          S[(size_t)(c0 + acc_row(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds] = acc[bm][bn][r];
+          S[(size_t)(c0 + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds] = acc[bm][bn][r];
     return;
   }
   // X into Bs (k-major B operand: Bs[kk*pitch + n] = X(kk, n))
@@ -182,7 +182,7 @@ __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* 
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Bs[acc_row(bm, r) * (TILE + 1) + acc_col(bn)] = acc[bm][bn][r];
+      for (int r = 0; r < 4; ++r) Bs[acc_row<T>(bm, r) * (TILE + 1) + acc_col(bn)] = acc[bm][bn][r];
   const int64_t rw = c0 + (int64_t)TILE * w;
   load_block_kmajor<T, T>(As, TILE + 1, L, ld, rw, c0);
   __syncthreads();
@@ -194,7 +194,7 @@ __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* 
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        T* p = &S[(size_t)(rw + acc_row(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds];
+        T* p = &S[(size_t)(rw + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds];
         *p = *p - acc[bm][bn][r];
       }
 }
