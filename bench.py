@@ -1,0 +1,309 @@
+#!/usr/bin/env python3
+"""GP posteriors/sec of the batched mixed-precision serving path on one MI355X.
+
+Workload: the serving request stream (seeded synthetic data): N log-uniform
+on [64, 4096], D uniform on {1..8}, X ~ U(-1, 1)^D, Gaussian (noise std 0.2)
+or ordinal probit (J uniform on {2..5}), EQ or Matern12 with lengthscale
+log-uniform on [0.2, 2], 256 test points per request.  One posterior =
+LaplaceGP.approximate_posterior + predict for one request.
+
+A step = one window of --requests-per-step requests served to completion by
+the continuous-batching engine (closed loop: the whole window is offered at
+the start of the step; requests enter as slots and memory free up).  All
+request inputs are resident in HBM before timing starts.  Also measured,
+outside the timed region:
+  * the plain per-request FP64 path (pgp_laplace_fit + pgp_predict, one
+    request at a time) on a sample of the same requests, and the maximum
+    relative predictive-mean error of the batched path against it;
+  * open-loop latency (Poisson arrivals) p50 / p99;
+  * the trailing-update kernel's roofline from HIP events over the timed
+    region (pgp_timer_*), and the CPU oracle baseline on a bounded sample.
+Prints ONE JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6}
+HBM_GBPS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=4)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--requests-per-step", type=int, default=512)
+    p.add_argument("--precision", default="fp32", choices=["bf16", "fp32", "fp64"])
+    p.add_argument("--capacity", type=int, default=256)
+    p.add_argument("--plain-sample", type=int, default=128)
+    p.add_argument("--open-loop", type=int, default=512, help="requests in the latency run")
+    p.add_argument("--rate", type=float, default=0.0,
+                   help="open-loop arrival rate (req/s); 0 = 0.7 x measured throughput")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--seed", type=int, default=4)
+    p.add_argument("--stream", type=int, default=4096)
+    return p.parse_args()
+
+
+def upload(reqs, torch):
+    out = []
+    for r in reqs:
+        p = r.problem
+        out.append(dict(rid=r.rid, arrival=r.arrival, p=p,
+                        X=torch.as_tensor(p.X, device="cuda"),
+                        y=torch.as_tensor(p.y, device="cuda"),
+                        Xt=torch.as_tensor(p.Xtest, device="cuda")))
+    return out
+
+
+def serve_window(engine, window, t_arrive=None):
+    """Serve a window to completion; returns {rid: (latency_s, Completed)}."""
+    import time as _t
+    pending = list(window)
+    t0 = _t.perf_counter()
+    res = {}
+    while pending or engine.in_flight:
+        still = []
+        for r in pending:
+            p = r["p"]
+            if t_arrive is not None and t_arrive[r["rid"]] > _t.perf_counter():
+                still.append(r)
+                continue
+            ok = engine.submit(r["rid"], r["X"], r["y"], p.kind, p.lengthscale, p.variance, p.lik,
+                               p.noise_std, p.cutpoints, r["Xt"], p.period)
+            if not ok:
+                still.append(r)
+        pending = still
+        if engine.in_flight == 0:
+            continue
+        for c in engine.step():
+            t_done = _t.perf_counter()
+            start = t0 if t_arrive is None else t_arrive[c.tag]
+            res[c.tag] = (t_done - start, c)
+    return res
+
+
+def plain_fp64(sample, torch, _lib):
+    """One request at a time through the single-problem entry points (FP64)."""
+    from probit_amd.kernels import EQ, Matern12
+    from probit_amd.utilities import likelihood_struct
+    lib = _lib.lib()
+    stream = _lib.stream_handle()
+    out = {}
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in sample:
+        ts = time.perf_counter()
+        p = r["p"]
+        N, D = p.X.shape
+        M = p.Xtest.shape[0]
+        base = EQ() if p.kind == "EQ" else Matern12()
+        ks = (p.variance * base.stretch(p.lengthscale)).to_struct(D)
+        kl = _lib.LIK_GAUSSIAN if p.lik == "gaussian" else _lib.LIK_PROBIT
+        ls = likelihood_struct(kl, p.likelihood_parameters())
+        ws = _lib.workspace(lib.pgp_workspace_size(N, D, 0), "cuda")
+        pws = _lib.workspace(lib.pgp_predict_workspace_size(N, M, 0), "cuda")
+        w = torch.empty(N, dtype=torch.float64, device="cuda")
+        prec = torch.empty_like(w)
+        mean = torch.empty(M, dtype=torch.float64, device="cuda")
+        var = torch.empty_like(mean)
+        prob = _lib.ProblemT(N, D, 0, r["X"].data_ptr(), r["y"].data_ptr(), w.data_ptr(),
+                             prec.data_ptr(), None)
+        opts = _lib.FitOptionsT(1e-5, 100, 0, 0, 0)
+        res = _lib.FitResultT()
+        _lib.check(lib.pgp_laplace_fit(prob, ks, ls, opts, res, ws.data_ptr(), ws.numel(),
+                                       stream), "fit")
+        _lib.check(lib.pgp_predict(ks, r["Xt"].data_ptr(), M, mean.data_ptr(), var.data_ptr(),
+                                   ws.data_ptr(), ws.numel(), pws.data_ptr(), pws.numel(),
+                                   stream), "predict")
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        out[r["rid"]] = (mean, var, res.iterations)
+    t1 = time.perf_counter()
+    return out, len(sample) / (t1 - t0), np.array(lat)
+
+
+def cpu_baseline(reqs, seconds):
+    """Reference-faithful oracle (dense Jacobian + LU Newton, dense predict
+    solve) on the host, over the stream in order until `seconds` elapse."""
+    from oracle import gp_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    done = 0
+    t0 = time.perf_counter()
+    nsum = 0
+    for r in reqs:
+        p = r.problem
+        kind = O.EQ if p.kind == "EQ" else O.MATERN12
+        spec = O.KernelSpec.make(kind, p.variance, p.lengthscale, p.D)
+        lik = O.GAUSSIAN if p.lik == "gaussian" else O.PROBIT
+        lp = p.likelihood_parameters()
+        K = O.gram(spec, p.X)
+        w, _, _ = O.laplace_weight(K, p.y, lik, lp, tol=1e-5)
+        W, _ = O.laplace_precision(K, p.y, lik, lp, w)
+        O.predict(spec, p.X, p.Xtest, w, W)
+        done += 1
+        nsum += p.N
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="posteriors/s", cores=int(cores), kind="port",
+                sample=f"first {done} requests of the same stream (mean N {nsum / max(done, 1):.0f}), "
+                       f"reference-faithful LU Newton + dense predict solve, NumPy FP64, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
+              "serving path shards by routing requests to independent per-GPU engines, "
+              "which is not run here", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from probit_amd import _lib, data
+    from probit_amd.serving import ServingEngine
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    lib = _lib.lib()
+
+    R, K, W = args.requests_per_step, args.steps, args.warmup
+    total = (K + W) * R
+    stream = data.serving_stream(seed=args.seed, n=min(args.stream, total))
+    order = [stream[i % len(stream)] for i in range(total)]
+    dev_reqs = upload(order, torch)
+    # distinct request ids even when the stream is cycled
+    for i, r in enumerate(dev_reqs):
+        r["rid"] = i
+    torch.cuda.synchronize()
+
+    eng = ServingEngine(precision=args.precision, capacity=args.capacity)
+    # warmup windows
+    for s in range(W):
+        serve_window(eng, dev_reqs[s * R:(s + 1) * R])
+    torch.cuda.synchronize()
+    lib.pgp_timer_reset()
+    lib.pgp_timer_enable(1 << _lib.TIMER_SYRK)
+    results = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(W, W + K):
+        results.update(serve_window(eng, dev_reqs[s * R:(s + 1) * R]))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.pgp_timer_enable(0)
+    import ctypes as C
+    n_l, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+    _lib.check(lib.pgp_timer_read(_lib.TIMER_SYRK, C.byref(n_l), C.byref(ms), C.byref(fl),
+                                  C.byref(by)), "timer")
+    elapsed = t1 - t0
+    value = K * R / elapsed
+    lat_closed = np.array([v[0] for v in results.values()])
+    iters = np.array([v[1].iterations for v in results.values()])
+
+    # plain FP64 per-request path on a sample of the timed requests
+    sample = dev_reqs[W * R:W * R + args.plain_sample]
+    plain, plain_rate, plain_lat = plain_fp64(sample, torch, _lib)
+    errs = []
+    for r in sample:
+        c = results[r["rid"]][1]
+        m_ref = plain[r["rid"]][0].cpu().numpy()
+        m = c.mean.cpu().numpy()
+        errs.append(float(np.max(np.abs(m - m_ref)) / max(1e-300, np.max(np.abs(m_ref)))))
+    errs = np.array(errs)
+
+    # open-loop latency at a fixed arrival rate
+    rate = args.rate if args.rate > 0 else 0.7 * value
+    ol = dev_reqs[W * R:W * R + min(args.open_loop, K * R)]
+    t_base = time.perf_counter() + 0.01
+    gaps = np.random.default_rng(args.seed + 1).exponential(1.0 / rate, len(ol))
+    arrive = {r["rid"]: t_base + float(t) for r, t in zip(ol, np.cumsum(gaps))}
+    open_res = serve_window(eng, ol, t_arrive=arrive)
+    lat_open = np.array([v[0] for v in open_res.values()])
+
+    cpu = cpu_baseline(order[W * R:], args.cpu_seconds)
+
+    ms_per_launch = ms.value / max(1, n_l.value)
+    achieved = fl.value / (ms.value * 1e-3) / 1e12 if ms.value > 0 else 0.0
+    peak = PEAK_TFLOPS[args.precision]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "syrk_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(args.precision)
+    line = {
+        "metric": "GP posteriors/sec (batched mixed-precision serving, p99 latency reported)",
+        "value": round(value, 2),
+        "unit": "posteriors/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * elapsed / K, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16": "bf16-mfma+fp32-acc+fp64", "fp32": "fp32-mfma+fp64",
+                  "fp64": "fp64"}[args.precision],
+        "data": "synthetic",
+        "config": {
+            "workload": "serving stream: N logU[64,4096], D U{1..8}, Gaussian / probit J U{2..5}, "
+                        "EQ / Matern12, 256 test points; approximate_posterior + predict",
+            "requests_per_step": R,
+            "capacity": args.capacity,
+            "precision": args.precision,
+            "tolerance": 1e-5,
+        },
+        "p50_latency_ms_closed_loop": round(1e3 * float(np.percentile(lat_closed, 50)), 3),
+        "p99_latency_ms_closed_loop": round(1e3 * float(np.percentile(lat_closed, 99)), 3),
+        "open_loop": {
+            "rate_req_per_s": round(rate, 1),
+            "requests": len(lat_open),
+            "p50_latency_ms": round(1e3 * float(np.percentile(lat_open, 50)), 3),
+            "p99_latency_ms": round(1e3 * float(np.percentile(lat_open, 99)), 3),
+        },
+        "plain_fp64": {
+            "value": round(plain_rate, 2),
+            "unit": "posteriors/s",
+            "requests": len(sample),
+            "p50_latency_ms": round(1e3 * float(np.percentile(plain_lat, 50)), 3),
+            "p99_latency_ms": round(1e3 * float(np.percentile(plain_lat, 99)), 3),
+        },
+        "speedup_vs_plain_fp64": round(value / plain_rate, 2),
+        "max_rel_mean_err_vs_fp64": float(np.max(errs)),
+        "mean_newton_iterations": round(float(np.mean(iters)), 2),
+        "roofline": {
+            "kernel": f"k_syrk ({args.precision} trailing update)",
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 5),
+            "traffic": traffic,
+            "launches": int(n_l.value),
+            "avg_launch_us": round(1e3 * ms_per_launch, 3),
+            "algo_flops_per_launch": fl.value / max(1, n_l.value),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -121,6 +121,17 @@ typedef struct pgp_problem {
 const char* pgp_last_error(void);
 int pgp_version(void);
 
+/* Opt-in launch timer for measurement: with bit c of `mask` set, launches of
+ * kernel class c (0 = trailing SYRK update, 1 = Cholesky panel, 2 = TRSV,
+ * 3 = Gram assembly, 4 = TRSM) are bracketed by HIP events on their stream,
+ * with the algorithmic flops / bytes each launch performs.  pgp_timer_read
+ * waits for the recorded events and returns the class totals.  Not
+ * thread-safe; no counterpart in the reference (measurement only). */
+int pgp_timer_enable(uint32_t mask);
+int pgp_timer_reset(void);
+int pgp_timer_read(int32_t cls, int64_t* launches, double* total_ms,
+                   double* algo_flops, double* algo_bytes);
+
 /* ------------------------------------------------------------------ */
 /* kernel-level entry points                                            */
 /* ------------------------------------------------------------------ */
@@ -149,12 +160,14 @@ int pgp_probit_predictive(const pgp_likelihood_t* lik, const double* mean,
                           const double* var, int64_t M, double* out, void* stream);
 
 /* In-place lower Cholesky A = L L^T of an N x N matrix (N % 64 == 0),
- * storage double (PGP_PREC_FP64) or float (FP32/BF16 modes).  logdiag
- * (N doubles, may be NULL) receives log L_ii computed in FP64; info (one
- * device int32) is set to PGP_INFO_NOT_PD on a non-positive pivot.
- * Replaces B.cholesky (probit/implicit/Laplace.py:24, VB.py:10,25). */
-int pgp_potrf(int precision, void* A, int64_t N, int64_t ld, double* logdiag,
-              int32_t* info, void* stream);
+ * storage double (PGP_PREC_FP64) or float (FP32/BF16 modes).  dinv
+ * (caller-owned, N * 64 doubles) receives the FP64 inverses of the 64 x 64
+ * diagonal blocks of L (what pgp_trsv / pgp_trsm recompute from L); logdiag
+ * (N doubles, may be NULL) receives log L_ii in FP64; info (one device int32)
+ * gets PGP_INFO_NOT_PD on a non-positive pivot.  Replaces B.cholesky
+ * (probit/implicit/Laplace.py:24, VB.py:10,25). */
+int pgp_potrf(int precision, void* A, int64_t N, int64_t ld, double* dinv,
+              double* logdiag, int32_t* info, void* stream);
 
 /* Triangular solve with the lower factor: trans = 0 solves L x = b,
  * trans = 1 solves L^T x = b.  dinv (caller-owned, N * 64 doubles) receives

@@ -31,6 +31,12 @@ INFO_NOT_PD = 1
 INFO_NEGATIVE_W = 2
 INFO_MAXITER = 4
 
+TIMER_SYRK = 0
+TIMER_PANEL = 1
+TIMER_TRSV = 2
+TIMER_GRAM = 3
+TIMER_TRSM = 4
+
 LIB_PATH = Path(__file__).resolve().parent / "libprobit_amd.so"
 
 
@@ -116,10 +122,14 @@ _I32 = C.c_int32
 _SIGNATURES = {
     "pgp_last_error": (C.c_char_p, []),
     "pgp_version": (C.c_int, []),
+    "pgp_timer_enable": (C.c_int, [C.c_uint32]),
+    "pgp_timer_reset": (C.c_int, []),
+    "pgp_timer_read": (C.c_int, [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pgp_gram": (C.c_int, [C.POINTER(KernelT), C.c_int, _P, _I64, _P, _I64, _P, _I64, _P]),
     "pgp_likelihood_eval": (C.c_int, [C.POINTER(LikelihoodT), _P, _P, _I64, _P, _P, _P, _P]),
     "pgp_probit_predictive": (C.c_int, [C.POINTER(LikelihoodT), _P, _P, _I64, _P, _P]),
-    "pgp_potrf": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _P]),
+    "pgp_potrf": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _P, _P]),
     "pgp_trsv": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _P, C.c_int, _P]),
     "pgp_trsm": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _I64, _I64, _P]),
     "pgp_workspace_size": (_I64, [_I64, _I32, _I32]),

@@ -336,6 +336,48 @@ static void fill_result_final(pgp_fit_result_t* r, const double* g4, int info) {
 
 static int default_refine(int prec) { return prec == PGP_PREC_FP64 ? 0 : (prec == PGP_PREC_FP32 ? 1 : 2); }
 
+// ---------------------------------------------------------------------------
+// opt-in launch timer (HIP events on the launching stream)
+// ---------------------------------------------------------------------------
+struct TimerRec {
+  hipEvent_t a, b;
+  double flops, bytes;
+  int cls;
+};
+struct TimerState {
+  unsigned mask = 0;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<TimerRec> recs;
+  hipEvent_t open[TC_COUNT] = {};
+};
+static TimerState& tstate() {
+  static TimerState t;
+  return t;
+}
+bool timer_on(int cls) { return (tstate().mask >> cls) & 1u; }
+static hipEvent_t timer_event() {
+  TimerState& t = tstate();
+  if (t.used == t.pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    t.pool.push_back(e);
+  }
+  return t.pool[t.used++];
+}
+void timer_begin(int cls, hipStream_t s) {
+  hipEvent_t e = timer_event();
+  if (e) (void)hipEventRecord(e, s);
+  tstate().open[cls] = e;
+}
+void timer_end(int cls, hipStream_t s, double flops, double bytes) {
+  hipEvent_t e = timer_event();
+  if (e) (void)hipEventRecord(e, s);
+  TimerState& t = tstate();
+  if (t.open[cls] && e) t.recs.push_back({t.open[cls], e, flops, bytes, cls});
+  t.open[cls] = nullptr;
+}
+
 }  // namespace pgp
 
 using namespace pgp;
@@ -343,6 +385,41 @@ using namespace pgp;
 extern "C" {
 
 const char* pgp_last_error(void) { return g_err.c_str(); }
+
+int pgp_timer_enable(uint32_t mask) {
+  tstate().mask = mask;
+  return 0;
+}
+
+int pgp_timer_reset(void) {
+  TimerState& t = tstate();
+  t.recs.clear();
+  t.used = 0;
+  return 0;
+}
+
+int pgp_timer_read(int32_t cls, int64_t* launches, double* total_ms, double* algo_flops,
+                   double* algo_bytes) {
+  if (cls < 0 || cls >= TC_COUNT) return fail(-1, "unknown timer class %d", cls);
+  TimerState& t = tstate();
+  int64_t n = 0;
+  double ms = 0.0, fl = 0.0, by = 0.0;
+  for (const TimerRec& r : t.recs) {
+    if (r.cls != cls) continue;
+    HIP_OK(hipEventSynchronize(r.b));
+    float dt = 0.f;
+    HIP_OK(hipEventElapsedTime(&dt, r.a, r.b));
+    ++n;
+    ms += dt;
+    fl += r.flops;
+    by += r.bytes;
+  }
+  if (launches) *launches = n;
+  if (total_ms) *total_ms = ms;
+  if (algo_flops) *algo_flops = fl;
+  if (algo_bytes) *algo_bytes = by;
+  return 0;
+}
 int pgp_version(void) { return 1; }
 
 int pgp_gram(const pgp_kernel_t* kernel, int dtype, const double* X, int64_t N, const double* Z,
@@ -378,14 +455,15 @@ int pgp_probit_predictive(const pgp_likelihood_t* lik, const double* mean, const
   return check_launch();
 }
 
-int pgp_potrf(int precision, void* A, int64_t N, int64_t ld, double* logdiag, int32_t* info,
-              void* stream) {
+int pgp_potrf(int precision, void* A, int64_t N, int64_t ld, double* dinv, double* logdiag,
+              int32_t* info, void* stream) {
   if (precision < 0 || precision > 2) return fail(-1, "unknown precision %d", precision);
   if (!A) return fail(-2, "A is NULL");
   if (N <= 0 || N % TILE) return fail(-3, "N must be a positive multiple of 64");
   if (ld < N) return fail(-4, "ld < N");
-  if (!info) return fail(-6, "info must be a device int32");
-  launch_potrf_raw(precision, A, N, ld, logdiag, info, (hipStream_t)stream);
+  if (!dinv) return fail(-5, "dinv must be a device buffer of N * 64 doubles");
+  if (!info) return fail(-7, "info must be a device int32");
+  launch_potrf_raw(precision, A, N, ld, dinv, logdiag, info, (hipStream_t)stream);
   return check_launch();
 }
 
@@ -765,6 +843,7 @@ struct pgp_batcher {
   std::vector<double> hg;
   std::vector<pgp::BItem> items;
   std::vector<pgp::BItem> pending;
+  std::vector<int> hnp[pgp::BATCH_ARRAYS];
 };
 
 namespace pgp {
@@ -776,9 +855,12 @@ static int64_t batcher_table_bytes(int cap) {
 static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v) {
   std::vector<Prob*> ptrs;
   ptrs.reserve(v.size());
+  std::vector<int>& hnp = bt->hnp[which];
+  hnp.clear();
   int maxNp = 0;
   for (BItem* b : v) {
     ptrs.push_back((Prob*)(b->ws + OFF_PROB));
+    hnp.push_back(b->p.Np);
     maxNp = std::max(maxNp, (int)b->p.Np);
   }
   write_ptrs(bt->arr[which], ptrs, bt->s);
@@ -786,6 +868,7 @@ static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v
   b.dprobs = bt->arr[which];
   b.count = (int)v.size();
   b.maxNp = maxNp;
+  b.hnp = hnp.data();
   return b;
 }
 

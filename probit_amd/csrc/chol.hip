@@ -3,17 +3,20 @@
 // solve of the Newton system (probit/implicit/solvers.py:24), whose matrix
 // -(I + W K) is factored here in the symmetric form B = I + W^1/2 K W^1/2.
 //
-// Step k (column block c0 = 64k) is two launches over all batch members:
-//   1. panel: every workgroup factors the 64x64 diagonal block in FP64
-//      registers (redundantly) together with one 64-row block below it, so
-//      the panel TRSM L21 = A21 L11^-T is fused into the factorisation;
-//      workgroup 0 writes L11, log(L_ii) and the FP64 pivots.
-//   2. trailing update A22 -= L21 L21^T on lower 64x64 tiles with MFMA:
-//      FP64 (v_mfma_f64_16x16x4), FP32 (v_mfma_f32_16x16x4) or BF16
-//      operands with FP32 accumulation (v_mfma_f32_16x16x32_bf16).
-// After the last step one launch inverts every 64x64 diagonal block (FP64),
-// which turns the diagonal solves of TRSV / TRSM into small products.
+// Schedule for every batch member at once (c0 = 64k):
+//   F    : factor diagonal block 0 (one workgroup per member);
+//   P_k  : panel  L_ik = A_ik L_kk^-T for all row blocks i > k, as an FP64
+//          MFMA product with the stored inverse of L_kk;
+//   S_k  : trailing update A_ij -= L_ik L_jk^T on lower 64x64 tiles with MFMA
+//          (FP64 / FP32 / BF16 operands with FP32 accumulation); the
+//          workgroup owning tile (k+1, k+1) then factors that diagonal block.
+// A diagonal block is factored by one workgroup in FP64 together with its
+// inverse (blocked by 16: each 16x16 diagonal sub-block inside one wave, the
+// sub-panels and updates by the whole workgroup), so the serial pivot chain
+// runs once per block and overlaps the other trailing tiles; the FP64
+// inverses (DINV) serve the panel products and the triangular solves.
 #include <math.h>
+#include <stdlib.h>
 
 #include "mfma.hpp"
 #include "pgp_internal.hpp"
@@ -28,164 +31,291 @@ __device__ __forceinline__ void lower_tile_c(int t, int& ti, int& tj) {
   tj = t - i * (i + 1) / 2;
 }
 
-// Factor the 128 x 64 panel [A11; A21] (A21 = rows r1..r1+63, absent if r1 < 0).
-// Thread (tr = t & 15, tc = t >> 4) holds rows 8tr..8tr+7, cols 4tc..4tc+3.
-template <typename T>
-__device__ void panel_factor(T* A, int64_t ld, int64_t c0, int64_t r1, bool write_diag,
-                             double* logdiag, double* diagval, int32_t* info) {
-  __shared__ double colbuf[128];
-  __shared__ double piv[64];
-  const int t = threadIdx.x;
-  const int tr = t & 15;
-  const int tc = t >> 4;
-  double a[8][4];
+struct FactorOut {
+  void* L;          // factor storage (T), ld
+  int64_t ld;
+  double* dinv;     // 64x64 FP64 inverse of this block, column-major
+  double* logdiag;  // 64 entries for this block (may be null)
+  double* diagval;  // 64 entries (may be null)
+  int32_t* info;
+};
+
+// 1 / sqrt(p) in FP64: hardware estimate refined by two Newton steps
+// r <- r (3 - p r^2) / 2 (each step doubles the correct bits).
+__device__ __forceinline__ double rsqrt_f64(double p) {
+  double r = __builtin_amdgcn_rsq(p);
+  r = r * (1.5 - 0.5 * p * r * r);
+  return r * (1.5 - 0.5 * p * r * r);
+}
+
+// Broadcast lane `lane`'s double to the whole wave (through scalar registers).
+__device__ __forceinline__ double bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// 16x16x16 FP64 MFMA product on LDS operands ([row][65] arrays), one wave:
+// acc(i, j) += sum_k A(ar + i, ac + k) * B(k, j), with B(k, j) = Bm(br + k, bc + j)
+// (bt = 0) or Bm(br + j, bc + k) (bt = 1).  acc[r] holds D(row (l >> 4) + 4r,
+// col l & 15) (lane l).
+__device__ __forceinline__ void mma16(const double (*A)[TILE + 1], int ar, int ac,
+                                      const double (*Bm)[TILE + 1], int br, int bc, int bt,
+                                      doublex4& acc) {
+  const int l = threadIdx.x & 63;
+  const int lr = l & 15, lk = l >> 4;
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int R = 8 * tr + r;
-    const bool valid = (R < 64) || (r1 >= 0);
-    const int64_t grow = (R < 64) ? c0 + R : r1 + (R - 64);
-#pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
-      const int64_t col = c0 + 4 * tc + cc;
-      a[r][cc] = valid ? (double)A[(size_t)grow + (size_t)col * ld] : 0.0;
-    }
-  }
-  // pivot (0, 0) is held by thread 0 as a[0][0]
-  if (t == 0) piv[0] = a[0][0];
-  __syncthreads();
-  for (int j = 0; j < 64; ++j) {
-    if (tc == (j >> 2)) {
-      const double p = piv[j];
-      const double d = sqrt(p);
-      const double inv = 1.0 / d;
-      const int jc = j & 3;
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int R = 8 * tr + r;
-        double cur = 0.0;
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
-          if (cc == jc) cur = a[r][cc];
-        double v;
-        if (R < j)
-          v = 0.0;
-        else if (R == j)
-          v = d;
-        else
-          v = cur * inv;
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
-          if (cc == jc && R >= j) a[r][cc] = v;
-        colbuf[R] = v;
-        if (R == j && write_diag) {
-          if (!(p > 0.0) && info) atomicOr(info, PGP_INFO_NOT_PD);
-          if (logdiag) logdiag[c0 + j] = log(d);
-          if (diagval) diagval[c0 + j] = d;
-        }
-      }
-    }
-    __syncthreads();
-    double cb[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) cb[r] = colbuf[8 * tr + r];
-#pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
-      const int m = 4 * tc + cc;
-      if (m > j) {
-        const double cm = colbuf[m];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) a[r][cc] -= cb[r] * cm;
-      }
-    }
-    if (j + 1 < 64 && tr == ((j + 1) >> 3) && tc == ((j + 1) >> 2)) {
-      const int rr = (j + 1) & 7, c1 = (j + 1) & 3;
-#pragma unroll
-      for (int r = 0; r < 8; ++r)
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
-          if (r == rr && cc == c1) piv[j + 1] = a[r][cc];
-    }
-    __syncthreads();
-  }
-  // write back
-#pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int R = 8 * tr + r;
-    if (R < 64) {
-      if (!write_diag) continue;
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc) {
-        const int m = 4 * tc + cc;
-        if (m <= R) A[(size_t)(c0 + R) + (size_t)(c0 + m) * ld] = (T)a[r][cc];
-      }
-    } else if (r1 >= 0) {
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc) {
-        const int m = 4 * tc + cc;
-        A[(size_t)(r1 + R - 64) + (size_t)(c0 + m) * ld] = (T)a[r][cc];
-      }
-    }
+  for (int kk = 0; kk < 16; kk += 4) {
+    const double a = A[ar + lr][ac + kk + lk];
+    const double b = bt ? Bm[br + lr][bc + kk + lk] : Bm[br + kk + lk][bc + lr];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
   }
 }
 
+// Factor the 16x16 diagonal sub-block at (o, o) of A in place (lower part)
+// and write its inverse to X(o.., o..) (zero above the diagonal).  One wave;
+// lane i (mod 16) holds row i in registers.  Each pivot goes through scalar
+// registers (readlane) and each scaled column through LDS, read back at one
+// address per step (a broadcast), so no block barrier is involved.
+__device__ __forceinline__ void factor16(double (*A)[TILE + 1], double (*X)[TILE + 1], int o,
+                                         double* dval, double* rinv, double* col,
+                                         int32_t* info) {
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 15;
+  double a[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) a[m] = A[o + i][o + m];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double p = bcast(a[j], j);
+    const double ri = rsqrt_f64(p);
+    const double d = p * ri;
+    bad |= !(p > 0.0);
+    const double lj = (i > j) ? a[j] * ri : (i == j ? d : 0.0);
+    if (i >= j) a[j] = lj;
+    col[16 * (j & 1) + i] = lj;  // all lanes of row i write the same value
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int m = j + 1; m < 16; ++m) a[m] -= lj * col[16 * (j & 1) + m];
+    if (lane == 0) {
+      dval[o + j] = d;
+      rinv[j] = ri;
+    }
+  }
+  if (lane == 0 && bad && info) atomicOr(info, PGP_INFO_NOT_PD);
+  if (lane < 16) {
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+      if (m <= i) A[o + i][o + m] = a[m];
+  }
+  __builtin_amdgcn_wave_barrier();
+  // inverse by columns: lane c runs forward substitution on column c of I;
+  // L entries are read back from LDS (one address per step: a broadcast)
+  const int c = i;
+  double x[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    double acc0 = (r == c) ? 1.0 : 0.0, acc1 = 0.0;
+#pragma unroll
+    for (int k = 0; k + 1 < r; k += 2) {
+      acc0 -= A[o + r][o + k] * x[k];
+      acc1 -= A[o + r][o + k + 1] * x[k + 1];
+    }
+    if (r & 1) acc0 -= A[o + r][o + r - 1] * x[r - 1];
+    x[r] = (acc0 + acc1) * rinv[r];
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) X[o + r][o + c] = (r >= c) ? x[r] : 0.0;
+  }
+}
+
+// Blocked factor + inverse of a 64x64 SPD block held in LDS (`A`, [row][65]).
+// Per 16-wide sub-panel s: one wave factors (and inverts) the diagonal
+// sub-block, then FP64 MFMA products form the sub-panel L_rs = A_rs X_ss^T
+// and the trailing update A_rc -= L_rs L_cs^T.  The off-diagonal blocks of
+// X = L^-1 follow by block distance: X_rs = -X_rr sum_{k=s}^{r-1} L_rk X_ks.
+// L (lower) goes to global storage as T, X to o.dinv (FP64).
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_potrf_panel(Prob* const* probs, int k) {
+__device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], double (*X)[TILE + 1], int64_t r0,
+                                     const FactorOut& o) {
+  __shared__ double dval[TILE], rinv[16], col[32];
+  const int t = threadIdx.x;
+  const int wv = t >> 6, l = t & 63;
+  const int orow = l >> 4, ocol = l & 15;  // D layout: row orow + 4r, col ocol
+  __builtin_amdgcn_s_setprio(3);
+  for (int s = 0; s < 4; ++s) {
+    const int o16 = 16 * s;
+    if (wv == 0) factor16(A, X, o16, dval, rinv, col, o.info);
+    __syncthreads();
+    const int nrb = 3 - s;
+    if (wv < nrb) {  // sub-panel block rb: L_rb,s = A_rb,s X_ss^T
+      const int rb = s + 1 + wv;
+      doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+      mma16(A, 16 * rb, o16, X, o16, o16, 1, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) A[16 * rb + orow + 4 * r][o16 + ocol] = acc[r];
+    }
+    __syncthreads();
+    // trailing blocks (r, c), s < c <= r: A_rc -= L_rs L_cs^T
+    for (int b = wv; b < nrb * (nrb + 1) / 2; b += 4) {
+      int rr = 0;
+      while ((rr + 1) * (rr + 2) / 2 <= b) ++rr;
+      const int cc = b - rr * (rr + 1) / 2;
+      const int r = s + 1 + rr, c = s + 1 + cc;
+      doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+      mma16(A, 16 * r, o16, A, 16 * c, o16, 1, acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) A[16 * r + orow + 4 * q][16 * c + ocol] -= acc[q];
+    }
+    __syncthreads();
+  }
+  // off-diagonal blocks of X by distance d; T goes to A's upper block (s, r)
+  for (int d = 1; d < 4; ++d) {
+    const int nblk = 4 - d;
+    if (wv < nblk) {
+      const int r = d + wv, s = wv;
+      doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k = s; k < r; ++k) mma16(A, 16 * r, 16 * k, X, 16 * k, 16 * s, 0, acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) A[16 * s + orow + 4 * q][16 * r + ocol] = acc[q];
+    }
+    __syncthreads();
+    if (wv < nblk) {
+      const int r = d + wv, s = wv;
+      doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+      mma16(X, 16 * r, 16 * r, A, 16 * s, 16 * r, 0, acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) X[16 * r + orow + 4 * q][16 * s + ocol] = -acc[q];
+    }
+    __syncthreads();
+  }
+  __builtin_amdgcn_s_setprio(0);
+  if (t < TILE) {
+    if (o.logdiag) o.logdiag[t] = log(dval[t]);
+    if (o.diagval) o.diagval[t] = dval[t];
+  }
+  T* L = (T*)o.L;
+  for (int e = t; e < TILE * TILE; e += NTHREADS) {
+    const int c = e / TILE, r = e % TILE;  // column-major walk: coalesced stores
+    if (c <= r) L[(size_t)(r0 + r) + (size_t)(r0 + c) * o.ld] = (T)A[r][c];
+    o.dinv[r + c * TILE] = (c <= r) ? X[r][c] : 0.0;
+  }
+}
+
+// ---- F: first diagonal block -------------------------------------------------
+template <typename T>
+__device__ void factor_first(const T* A, int64_t ld, const FactorOut& o) {
+  __shared__ double blk[2][TILE][TILE + 1];
+  const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
+  for (int q = 0; q < 16; ++q) {
+    const int c = kb + 4 * q;
+    blk[0][m][c] = (double)A[(size_t)m + (size_t)c * ld];
+  }
+  __syncthreads();
+  factor_block_blocked<T>(blk[0], blk[1], 0, o);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_factor_first(Prob* const* probs) {
+  const Prob& P = *probs[blockIdx.y];
+  FactorOut o{P.L, P.Np, P.dinv, P.logdiag, P.diagval, P.info};
+  factor_first<T>((const T*)P.L, P.Np, o);
+}
+
+// ---- P_k: panel rows below block k --------------------------------------------
+template <typename T>
+__device__ void panel_rows(T* A, int64_t ld, const double* dinv_k, int64_t c0, int64_t r0) {
+  __shared__ double As[TILE * (TILE + 1)];
+  __shared__ double Bs[TILE * (TILE + 1)];
+  // As[p][m] = A(r0 + m, c0 + p); Bs[p][n] = DINV_k(n, p)
+  load_block_kmajor<T, double>(As, TILE + 1, A, ld, r0, c0);
+  {
+    const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
+    for (int q = 0; q < 16; ++q) {
+      const int p = pb + 4 * q;
+      Bs[p * (TILE + 1) + n] = dinv_k[n + p * TILE];
+    }
+  }
+  __syncthreads();
+  doublex4 acc[2][2];
+  zero_acc<double>(acc);
+  mma64_kmajor<double>(As, TILE + 1, Bs, TILE + 1, acc);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        A[(size_t)(r0 + acc_row<double>(bm, r)) + (size_t)(c0 + acc_col(bn)) * ld] =
+            (T)acc[bm][bn][r];
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_panel(Prob* const* probs, int k) {
   const Prob& P = *probs[blockIdx.y];
   const int64_t c0 = (int64_t)k * TILE;
-  if (c0 >= P.Np) return;
-  const int nbk = (int)((P.Np - c0) / TILE);
-  const int w = blockIdx.x;
-  if (w >= (nbk > 1 ? nbk - 1 : 1)) return;
-  const int64_t r1 = (nbk > 1) ? c0 + (int64_t)TILE * (w + 1) : -1;
-  panel_factor<T>((T*)P.L, P.Np, c0, r1, w == 0, P.logdiag, P.diagval, P.info);
+  const int64_t r0 = c0 + (int64_t)TILE * (blockIdx.x + 1);
+  if (r0 >= P.Np) return;
+  panel_rows<T>((T*)P.L, P.Np, P.dinv + (size_t)k * TILE * TILE, c0, r0);
 }
 
-template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_potrf_panel_raw(T* A, int64_t N, int64_t ld, int k,
-                                                              double* logdiag, int32_t* info) {
-  const int64_t c0 = (int64_t)k * TILE;
-  const int nbk = (int)((N - c0) / TILE);
-  const int w = blockIdx.x;
-  const int64_t r1 = (nbk > 1) ? c0 + (int64_t)TILE * (w + 1) : -1;
-  panel_factor<T>(A, ld, c0, r1, w == 0, logdiag, nullptr, info);
-}
-
-// ---- trailing update of one lower tile: C(ri.., rj..) -= P_i P_j^T ----
-template <typename T, int BF16>
-__device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj) {
+// ---- S_k: trailing update; tile (k+1, k+1) then factors its block ----------
+template <typename T, int BF16, int FUSE>
+__device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, bool factor,
+                          const FactorOut& o) {
+  // one LDS arena: MMA operands, then (fused mode, tile (k+1, k+1) only) the
+  // two FP64 blocks of the diagonal factorisation
+  constexpr int OPER = BF16 ? 2 * TILE * 72 * 2 : 2 * TILE * (TILE + 1) * (int)sizeof(T);
+  constexpr int BLK = FUSE ? 2 * TILE * (TILE + 1) * 8 : 0;
+  __shared__ __attribute__((aligned(16))) char arena[OPER > BLK ? OPER : BLK];
   typedef typename Acc<T>::type acc_t;
   acc_t acc[2][2];
   zero_acc<T>(acc);
   if constexpr (BF16) {
-    __shared__ __attribute__((aligned(16))) unsigned short si[TILE * 72];
-    __shared__ __attribute__((aligned(16))) unsigned short sj[TILE * 72];
+    unsigned short* si = (unsigned short*)arena;
+    unsigned short* sj = si + TILE * 72;
     load_block_bf16_mmajor<T>(si, 72, A, ld, ri, c0);
     load_block_bf16_mmajor<T>(sj, 72, A, ld, rj, c0);
     __syncthreads();
-    mma64_bf16(si, 72, sj, 72, *reinterpret_cast<floatx4(*)[2][2]>(&acc));
+    mma64_bf16(si, 72, sj, 72, acc);
   } else {
-    __shared__ T si[TILE * (TILE + 1)];
-    __shared__ T sj[TILE * (TILE + 1)];
+    T* si = (T*)arena;
+    T* sj = si + TILE * (TILE + 1);
     load_block_kmajor<T, T>(si, TILE + 1, A, ld, ri, c0);
     load_block_kmajor<T, T>(sj, TILE + 1, A, ld, rj, c0);
     __syncthreads();
     mma64_kmajor<T>(si, TILE + 1, sj, TILE + 1, acc);
   }
+  if constexpr (!FUSE) factor = false;
+  double (*blk)[TILE + 1] = (double (*)[TILE + 1])arena;
+  if (factor) __syncthreads();  // operands are dead; the arena becomes the block
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn) {
-      const int64_t col = rj + acc_col(bn);
+      const int col = acc_col(bn);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = ri + acc_row<T>(bm, r);
-        T* p = &A[(size_t)row + (size_t)col * ld];
-        *p = *p - acc[bm][bn][r];
+        const int row = acc_row<T>(bm, r);
+        T* p = &A[(size_t)(ri + row) + (size_t)(rj + col) * ld];
+        const T v = *p - acc[bm][bn][r];
+        if (factor)
+          blk[row][col] = (double)v;
+        else
+          *p = v;
       }
     }
+  if constexpr (FUSE) {
+    if (factor) {
+      __syncthreads();
+      factor_block_blocked<T>(blk, blk + TILE, ri, o);
+    }
+  }
 }
 
-template <typename T, int BF16>
+template <typename T, int BF16, int FUSE>
 __global__ __launch_bounds__(NTHREADS) void k_syrk(Prob* const* probs, int k) {
   const Prob& P = *probs[blockIdx.y];
   const int64_t c0 = (int64_t)k * TILE, c1 = c0 + TILE;
@@ -194,20 +324,65 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk(Prob* const* probs, int k) {
   if ((int)blockIdx.x >= Tk * (Tk + 1) / 2) return;
   int ti, tj;
   lower_tile_c(blockIdx.x, ti, tj);
-  syrk_tile<T, BF16>((T*)P.L, P.Np, c0, c1 + (int64_t)TILE * ti, c1 + (int64_t)TILE * tj);
+  const bool factor = (ti == 0 && tj == 0);
+  FactorOut o{P.L, P.Np, P.dinv + (size_t)(k + 1) * TILE * TILE,
+              P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr, P.info};
+  syrk_tile<T, BF16, FUSE>((T*)P.L, P.Np, c0, c1 + (int64_t)TILE * ti, c1 + (int64_t)TILE * tj,
+                           factor, o);
+}
+
+// Diagonal block k (k >= 1) after the trailing update of step k - 1 (separate
+// mode): one workgroup per member.
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_factor_diag(Prob* const* probs, int k) {
+  const Prob& P = *probs[blockIdx.y];
+  const int64_t c0 = (int64_t)k * TILE;
+  if (c0 >= P.Np) return;
+  __shared__ double blk[2][TILE][TILE + 1];
+  const T* A = (const T*)P.L;
+  const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
+  for (int q = 0; q < 16; ++q) {
+    const int c = kb + 4 * q;
+    blk[0][m][c] = (double)A[(size_t)(c0 + m) + (size_t)(c0 + c) * P.Np];
+  }
+  __syncthreads();
+  FactorOut o{P.L, P.Np, P.dinv + (size_t)k * TILE * TILE, P.logdiag ? P.logdiag + c0 : nullptr,
+              P.diagval ? P.diagval + c0 : nullptr, P.info};
+  factor_block_blocked<T>(blk[0], blk[1], c0, o);
+}
+
+// ---- raw (kernel-level ABI) variants ----------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_factor_first_raw(T* A, int64_t ld, double* dinv,
+                                                               double* logdiag, int32_t* info) {
+  FactorOut o{A, ld, dinv, logdiag, nullptr, info};
+  factor_first<T>(A, ld, o);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_panel_raw(T* A, int64_t N, int64_t ld,
+                                                        const double* dinv, int k) {
+  const int64_t c0 = (int64_t)k * TILE;
+  const int64_t r0 = c0 + (int64_t)TILE * (blockIdx.x + 1);
+  panel_rows<T>(A, ld, dinv + (size_t)k * TILE * TILE, c0, r0);
 }
 
 template <typename T, int BF16>
-__global__ __launch_bounds__(NTHREADS) void k_syrk_raw(T* A, int64_t N, int64_t ld, int k) {
+__global__ __launch_bounds__(NTHREADS) void k_syrk_raw(T* A, int64_t N, int64_t ld, int k,
+                                                       double* dinv, double* logdiag,
+                                                       int32_t* info) {
   const int64_t c0 = (int64_t)k * TILE, c1 = c0 + TILE;
   int ti, tj;
   lower_tile_c(blockIdx.x, ti, tj);
-  syrk_tile<T, BF16>(A, ld, c0, c1 + (int64_t)TILE * ti, c1 + (int64_t)TILE * tj);
+  const bool factor = (ti == 0 && tj == 0);
+  FactorOut o{A, ld, dinv + (size_t)(k + 1) * TILE * TILE, logdiag ? logdiag + c1 : nullptr,
+              nullptr, info};
+  syrk_tile<T, BF16, 1>(A, ld, c0, c1 + (int64_t)TILE * ti, c1 + (int64_t)TILE * tj, factor, o);
 }
 
-// ---- inverse of every 64x64 diagonal block of the factor (FP64) ----
-// DINV block k: column-major 64x64 at dinv + k * 4096.  Lane c of wave 0..
-// computes column c of inv(L_kk) by forward substitution.
+// ---- inverse of every 64x64 diagonal block of a given factor (FP64) --------
+// Used when a caller hands in a factor (kernel-level solves).  Lane i of each
+// wave holds x_i of one column of inv(L_kk) during forward substitution.
 template <typename T>
 __device__ void invert_diag_block(const T* A, int64_t ld, int64_t c0, double* out) {
   __shared__ double Ls[TILE][TILE + 1];  // Ls[col][row]
@@ -217,18 +392,15 @@ __device__ void invert_diag_block(const T* A, int64_t ld, int64_t c0, double* ou
     Ls[col][m] = (m >= col) ? (double)A[(size_t)(c0 + m) + (size_t)(c0 + col) * ld] : 0.0;
   }
   __syncthreads();
-  // 4 waves x 16 columns each; lane = row index i of the solution column
   const int wv = threadIdx.x >> 6;
   const int i = threadIdx.x & 63;
   for (int cq = 0; cq < 16; ++cq) {
     const int c = wv * 16 + cq;
-    // solve L x = e_c : x_j = (delta_jc - sum_{p<j} L_jp x_p) / L_jj
-    double x = 0.0;  // lane i holds x_i
+    double x = 0.0;
     double rhs = (i == c) ? 1.0 : 0.0;
     for (int j = c; j < 64; ++j) {
-      double xj = 0.0;
       if (i == j) x = rhs / Ls[j][j];
-      xj = __shfl(x, j);
+      const double xj = __shfl(x, j);
       if (i > j) rhs -= Ls[j][i] * xj;
     }
     out[(size_t)i + (size_t)c * TILE] = (i >= c) ? x : 0.0;
@@ -236,31 +408,60 @@ __device__ void invert_diag_block(const T* A, int64_t ld, int64_t c0, double* ou
 }
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_dinv(Prob* const* probs) {
-  const Prob& P = *probs[blockIdx.y];
-  const int nb = P.Np / TILE;
-  if ((int)blockIdx.x >= nb) return;
-  invert_diag_block<T>((const T*)P.L, P.Np, (int64_t)blockIdx.x * TILE,
-                       P.dinv + (size_t)blockIdx.x * TILE * TILE);
-}
-
-template <typename T>
 __global__ __launch_bounds__(NTHREADS) void k_dinv_raw(const T* A, int64_t ld, double* dinv) {
   invert_diag_block<T>(A, ld, (int64_t)blockIdx.x * TILE, dinv + (size_t)blockIdx.x * TILE * TILE);
+}
+
+// Diagonal factorisation fused into the trailing-update tile that produces
+// the block (1) or run as its own launch after the update (0, the default:
+// it keeps the trailing-update kernel's register and LDS footprint small).
+// PGP_FUSED_FACTOR=1 selects the fused schedule.
+static bool fused_factor() {
+  static const bool v = [] {
+    const char* e = getenv("PGP_FUSED_FACTOR");
+    return e && e[0] == '1';
+  }();
+  return v;
 }
 
 // ---------------------------------------------------------------------------
 template <typename T, int BF16>
 static void potrf_steps(const Batch& b, hipStream_t s) {
   const int nb = b.maxNp / TILE;
-  for (int k = 0; k < nb; ++k) {
-    const int nbk = nb - k;
-    dim3 gp(nbk > 1 ? nbk - 1 : 1, b.count);
-    hipLaunchKernelGGL(k_potrf_panel<T>, gp, dim3(NTHREADS), 0, s, b.dprobs, k);
-    const int Tk = nbk - 1;
-    if (Tk > 0) {
-      dim3 gs(Tk * (Tk + 1) / 2, b.count);
-      hipLaunchKernelGGL((k_syrk<T, BF16>), gs, dim3(NTHREADS), 0, s, b.dprobs, k);
+  const bool tsy = timer_on(TC_SYRK), tpa = timer_on(TC_PANEL);
+  if (tpa) timer_begin(TC_PANEL, s);
+  hipLaunchKernelGGL(k_factor_first<T>, dim3(1, b.count), dim3(NTHREADS), 0, s, b.dprobs);
+  if (tpa) timer_end(TC_PANEL, s, b.count * (TILE * TILE * TILE / 3.0), 0.0);
+  for (int k = 0; k + 1 < nb; ++k) {
+    const int below = nb - k - 1;
+    // algorithmic work summed over members: panel n * 64^2, trailing
+    // n (n + 1) * 64 (lower triangle, rank 64), next diagonal block 64^3 / 3,
+    // with n the member's trailing size after this panel.
+    double fp = 0.0, fs = 0.0;
+    if (tsy || tpa) {
+      for (int m = 0; m < b.count; ++m) {
+        const double np = b.hnp ? b.hnp[m] : b.maxNp;
+        const double n = np - (double)TILE * (k + 1);
+        if (n <= 0) continue;
+        fp += n * TILE * TILE;
+        fs += n * (n + 1) * TILE + TILE * TILE * TILE / 3.0;
+      }
+    }
+    if (tpa) timer_begin(TC_PANEL, s);
+    hipLaunchKernelGGL(k_panel<T>, dim3(below, b.count), dim3(NTHREADS), 0, s, b.dprobs, k);
+    if (tpa) timer_end(TC_PANEL, s, fp, 0.0);
+    if (tsy) timer_begin(TC_SYRK, s);
+    if (fused_factor())
+      hipLaunchKernelGGL((k_syrk<T, BF16, 1>), dim3(below * (below + 1) / 2, b.count),
+                         dim3(NTHREADS), 0, s, b.dprobs, k);
+    else
+      hipLaunchKernelGGL((k_syrk<T, BF16, 0>), dim3(below * (below + 1) / 2, b.count),
+                         dim3(NTHREADS), 0, s, b.dprobs, k);
+    if (tsy) timer_end(TC_SYRK, s, fs, 0.0);
+    if (!fused_factor()) {
+      if (tpa) timer_begin(TC_PANEL, s);
+      hipLaunchKernelGGL(k_factor_diag<T>, dim3(1, b.count), dim3(NTHREADS), 0, s, b.dprobs, k + 1);
+      if (tpa) timer_end(TC_PANEL, s, b.count * (TILE * TILE * TILE / 3.0), 0.0);
     }
   }
 }
@@ -274,37 +475,37 @@ void launch_potrf(const Batch& b, int prec, hipStream_t s) {
     potrf_steps<float, 1>(b, s);
 }
 
+// The factorisation leaves DINV in place; kept as a no-op entry for callers
+// that re-factor through launch_potrf.
 void launch_dinv(const Batch& b, int prec, hipStream_t s) {
-  dim3 g(b.maxNp / TILE, b.count);
-  if (prec == PGP_PREC_FP64)
-    hipLaunchKernelGGL(k_dinv<double>, g, dim3(NTHREADS), 0, s, b.dprobs);
-  else
-    hipLaunchKernelGGL(k_dinv<float>, g, dim3(NTHREADS), 0, s, b.dprobs);
+  (void)b;
+  (void)prec;
+  (void)s;
 }
 
 template <typename T, int BF16>
-static void potrf_raw_steps(T* A, int64_t N, int64_t ld, double* logdiag, int32_t* info,
-                            hipStream_t s) {
+static void potrf_raw_steps(T* A, int64_t N, int64_t ld, double* dinv, double* logdiag,
+                            int32_t* info, hipStream_t s) {
   const int nb = (int)(N / TILE);
-  for (int k = 0; k < nb; ++k) {
-    const int nbk = nb - k;
-    hipLaunchKernelGGL(k_potrf_panel_raw<T>, dim3(nbk > 1 ? nbk - 1 : 1), dim3(NTHREADS), 0, s, A,
-                       N, ld, k, logdiag, info);
-    const int Tk = nbk - 1;
-    if (Tk > 0)
-      hipLaunchKernelGGL((k_syrk_raw<T, BF16>), dim3(Tk * (Tk + 1) / 2), dim3(NTHREADS), 0, s, A,
-                         N, ld, k);
+  hipLaunchKernelGGL(k_factor_first_raw<T>, dim3(1), dim3(NTHREADS), 0, s, A, ld, dinv, logdiag,
+                     info);
+  for (int k = 0; k + 1 < nb; ++k) {
+    const int below = nb - k - 1;
+    hipLaunchKernelGGL(k_panel_raw<T>, dim3(below), dim3(NTHREADS), 0, s, A, N, ld,
+                       (const double*)dinv, k);
+    hipLaunchKernelGGL((k_syrk_raw<T, BF16>), dim3(below * (below + 1) / 2), dim3(NTHREADS), 0, s,
+                       A, N, ld, k, dinv, logdiag, info);
   }
 }
 
-int launch_potrf_raw(int prec, void* A, int64_t N, int64_t ld, double* logdiag, int32_t* info,
-                     hipStream_t s) {
+int launch_potrf_raw(int prec, void* A, int64_t N, int64_t ld, double* dinv, double* logdiag,
+                     int32_t* info, hipStream_t s) {
   if (prec == PGP_PREC_FP64)
-    potrf_raw_steps<double, 0>((double*)A, N, ld, logdiag, info, s);
+    potrf_raw_steps<double, 0>((double*)A, N, ld, dinv, logdiag, info, s);
   else if (prec == PGP_PREC_FP32)
-    potrf_raw_steps<float, 0>((float*)A, N, ld, logdiag, info, s);
+    potrf_raw_steps<float, 0>((float*)A, N, ld, dinv, logdiag, info, s);
   else
-    potrf_raw_steps<float, 1>((float*)A, N, ld, logdiag, info, s);
+    potrf_raw_steps<float, 1>((float*)A, N, ld, dinv, logdiag, info, s);
   return 0;
 }
 

@@ -132,7 +132,18 @@ __global__ __launch_bounds__(NTHREADS) void k_gram_self(Prob* const* probs) {
 void launch_gram_self(const Batch& b, hipStream_t s) {
   const int T = b.maxNp / TILE;
   dim3 grid(T * (T + 1) / 2, b.count);
+  const bool on = timer_on(TC_GRAM);
+  if (on) timer_begin(TC_GRAM, s);
   hipLaunchKernelGGL(k_gram_self, grid, dim3(NTHREADS), 0, s, b.dprobs);
+  if (on) {
+    // algorithmic bytes: the Np x Np FP64 matrix written once
+    double bytes = 0.0;
+    for (int m = 0; m < b.count; ++m) {
+      const double np = b.hnp ? b.hnp[m] : b.maxNp;
+      bytes += np * np * 8.0;
+    }
+    timer_end(TC_GRAM, s, 0.0, bytes);
+  }
 }
 
 // ---- B = I + S K S (Laplace) or sigma^2 I + K (VB) into the factor storage ----

@@ -92,7 +92,15 @@ struct Batch {
   Prob* const* dprobs;  // device array of device pointers to descriptors
   int count;            // batch members in this launch
   int maxNp;            // max padded size among members
+  const int* hnp = nullptr;  // host copy of each member's Np (for launch accounting)
 };
+
+// Opt-in launch timer: HIP events around the launches of one kernel class
+// on the launching stream, with the algorithmic flops each launch performs.
+enum TimerClass { TC_SYRK = 0, TC_PANEL, TC_TRSV, TC_GRAM, TC_TRSM, TC_COUNT };
+void timer_begin(int cls, hipStream_t s);
+void timer_end(int cls, hipStream_t s, double algo_flops, double algo_bytes);
+bool timer_on(int cls);
 
 // gram.hip
 void launch_gram_self(const Batch& b, hipStream_t s);
@@ -113,8 +121,8 @@ int launch_probit_predictive(const pgp_likelihood_t* lik, const double* mean, co
 // chol.hip
 void launch_potrf(const Batch& b, int prec, hipStream_t s);
 void launch_dinv(const Batch& b, int prec, hipStream_t s);
-int launch_potrf_raw(int prec, void* A, int64_t N, int64_t ld, double* logdiag, int32_t* info,
-                     hipStream_t s);
+int launch_potrf_raw(int prec, void* A, int64_t N, int64_t ld, double* dinv, double* logdiag,
+                     int32_t* info, hipStream_t s);
 int launch_dinv_raw(int prec, const void* A, int64_t N, int64_t ld, double* dinv, hipStream_t s);
 
 // solve.hip

@@ -165,7 +165,7 @@ def serving_stream(seed=4, n=4096, rate=2000.0, M=256, Nmin=64, Nmax=4096):
         kind = EQ if rng.random() < 0.5 else MATERN12
         ls = float(math.exp(rng.uniform(math.log(0.2), math.log(2.0))))
         X = rng.uniform(-1.0, 1.0, (N + M, D))
-        f = prior_sample_rff(rng, kind, X, ls, 1.0, R=256)
+        f = prior_sample_rff(rng, kind, X, ls, 1.0, R=64)
         if rng.random() < 0.5:
             y = f[:N] + 0.2 * rng.standard_normal(N)
             p = Problem(X[:N], y, kind, ls, 1.0, "gaussian", 0.2, Xtest=X[N:])

@@ -81,18 +81,25 @@ class ServingEngine:
                 and self.used + self.request_bytes(N, D, M) <= self.budget)
 
     # -- requests ---------------------------------------------------------------
+    def _dev(self, a) -> torch.Tensor:
+        if isinstance(a, torch.Tensor) and a.device == self.device and a.dtype == torch.float64:
+            return a.contiguous()
+        if isinstance(a, torch.Tensor):
+            return a.to(device=self.device, dtype=torch.float64).contiguous()
+        return torch.as_tensor(np.asarray(a, dtype=np.float64), device=self.device)
+
     def submit(self, tag: int, X, y, kind: str, lengthscale, variance, lik: str, noise_std,
                cutpoints=None, Xtest=None, period=None) -> bool:
-        """Queue one posterior; returns False (nothing queued) if it does not fit now."""
-        X = np.asarray(X, dtype=np.float64)
-        N, D = X.shape
-        M = 0 if Xtest is None else int(np.asarray(Xtest).shape[0])
+        """Queue one posterior; returns False (nothing queued) if it does not fit now.
+        Device tensors are used in place; host arrays are copied to the GPU."""
+        N, D = tuple(X.shape)
+        M = 0 if Xtest is None else int(Xtest.shape[0])
         if not self.can_admit(N, D, M):
             return False
         dev = self.device
         t = {}
-        t["X"] = torch.as_tensor(X, device=dev)
-        t["y"] = torch.as_tensor(np.asarray(y, dtype=np.float64), device=dev)
+        t["X"] = self._dev(X)
+        t["y"] = self._dev(y)
         t["w"] = torch.empty(N, dtype=torch.float64, device=dev)
         t["p"] = torch.empty(N, dtype=torch.float64, device=dev)
         nb = self.lib.pgp_workspace_size(N, D, self.prec)
@@ -115,7 +122,7 @@ class ServingEngine:
         item.workspace = t["ws"].data_ptr()
         item.workspace_bytes = nb
         if M > 0:
-            t["Xt"] = torch.as_tensor(np.asarray(Xtest, dtype=np.float64), device=dev)
+            t["Xt"] = self._dev(Xtest)
             t["mean"] = torch.empty(M, dtype=torch.float64, device=dev)
             t["var"] = torch.empty(M, dtype=torch.float64, device=dev)
             pb = self.lib.pgp_predict_workspace_size(N, M, self.prec)

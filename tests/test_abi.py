@@ -51,8 +51,8 @@ def test_abi_rejects_bad_arguments_without_touching_the_gpu():
 
     from probit_amd import _lib
     lib = _lib.load()
-    assert lib.pgp_potrf(0, None, 64, 64, None, None, None) < 0
-    assert lib.pgp_potrf(7, None, 64, 64, None, None, None) < 0
+    assert lib.pgp_potrf(0, None, 64, 64, None, None, None, None) < 0
+    assert lib.pgp_potrf(7, None, 64, 64, None, None, None, None) < 0
     assert lib.pgp_laplace_fit(None, None, None, None, None, None, 0, None) < 0
     assert b"NULL" in lib.pgp_last_error() or len(lib.pgp_last_error()) > 0
     h = C.c_void_p()

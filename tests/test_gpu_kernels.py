@@ -115,8 +115,9 @@ def test_potrf_backward_error(prec, tol, N):
     Ad = torch.as_tensor(A, dtype=tdt, device="cuda").t().contiguous()  # column-major
     logd = torch.empty(N, dtype=torch.float64, device="cuda")
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    rc = lib.pgp_potrf(prec, Ad.data_ptr(), N, N, logd.data_ptr(), info.data_ptr(),
-                       _lib.stream_handle())
+    dinv = torch.empty(N * 64, dtype=torch.float64, device="cuda")
+    rc = lib.pgp_potrf(prec, Ad.data_ptr(), N, N, dinv.data_ptr(), logd.data_ptr(),
+                       info.data_ptr(), _lib.stream_handle())
     _lib.check(rc, "pgp_potrf")
     torch.cuda.synchronize()
     assert int(info.item()) == 0
@@ -127,6 +128,11 @@ def test_potrf_backward_error(prec, tol, N):
     if prec == 0:
         assert rel_err(L, Lref) < 1e-12
         assert rel_err(logd.cpu().numpy(), np.log(np.diag(Lref))) < 1e-12
+    Dv = dinv.cpu().numpy().reshape(N // 64, 64, 64)  # block k, column-major
+    for k in range(N // 64):
+        blkL = L[64 * k:64 * k + 64, 64 * k:64 * k + 64]
+        inv = Dv[k].T
+        assert np.max(np.abs(inv @ blkL - np.eye(64))) < (1e-12 if prec == 0 else 1e-4)
 
 
 def test_potrf_reports_not_pd():
@@ -136,7 +142,9 @@ def test_potrf_reports_not_pd():
     A[70, 70] = -1.0
     Ad = torch.as_tensor(A, device="cuda").t().contiguous()
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    rc = lib.pgp_potrf(0, Ad.data_ptr(), N, N, None, info.data_ptr(), _lib.stream_handle())
+    dinv = torch.empty(N * 64, dtype=torch.float64, device="cuda")
+    rc = lib.pgp_potrf(0, Ad.data_ptr(), N, N, dinv.data_ptr(), None, info.data_ptr(),
+                       _lib.stream_handle())
     _lib.check(rc, "pgp_potrf")
     torch.cuda.synchronize()
     assert int(info.item()) & _lib.INFO_NOT_PD
@@ -146,7 +154,8 @@ def test_potrf_rejects_bad_sizes():
     torch, _lib, lib = _setup()
     A = torch.zeros((100, 100), dtype=torch.float64, device="cuda")
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    rc = lib.pgp_potrf(0, A.data_ptr(), 100, 100, None, info.data_ptr(), _lib.stream_handle())
+    rc = lib.pgp_potrf(0, A.data_ptr(), 100, 100, A.data_ptr(), None, info.data_ptr(),
+                       _lib.stream_handle())
     assert rc < 0
 
 
