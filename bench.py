@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--requests-per-step", type=int, default=512)
     p.add_argument("--precision", default="fp32", choices=["bf16", "fp32", "fp64"])
     p.add_argument("--capacity", type=int, default=256)
+    p.add_argument("--refine", type=int, default=-1,
+                   help="FP64 refinement steps per Newton solve (-1: library default)")
     p.add_argument("--plain-sample", type=int, default=128)
     p.add_argument("--open-loop", type=int, default=512, help="requests in the latency run")
     p.add_argument("--rate", type=float, default=0.0,
@@ -194,7 +196,8 @@ def main():
         r["rid"] = i
     torch.cuda.synchronize()
 
-    eng = ServingEngine(precision=args.precision, capacity=args.capacity)
+    eng = ServingEngine(precision=args.precision, capacity=args.capacity,
+                        refine=None if args.refine < 0 else args.refine)
     # warmup windows
     for s in range(W):
         serve_window(eng, dev_reqs[s * R:(s + 1) * R])

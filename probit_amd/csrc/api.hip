@@ -67,6 +67,8 @@ constexpr int64_t OFF_ARR = 2048;   // pointer array (self) for single-problem l
 constexpr int64_t OFF_INFO = 2112;  // int32[4]
 constexpr int64_t OFF_RED = 2176;   // double[NRED]
 constexpr int64_t OFF_GATHER = 2432;  // double[8] scratch for reductions
+constexpr int64_t OFF_PROGRESS = 2496;  // uint64 progress word of the wavefront solves
+constexpr int64_t OFF_TICKET = 2504;    // uint32 workgroup-order counter (single-problem launches)
 constexpr int64_t HEADER = 4096;
 
 struct WsLayout {
@@ -139,6 +141,8 @@ __global__ void k_write_prob(Prob* dst, Prob val, int reset) {
   if (reset) {
     if (threadIdx.x < 4) val.info[threadIdx.x] = 0;
     if (threadIdx.x < NRED) val.red[threadIdx.x] = 0.0;
+    // progress word and ticket counter of the header
+    if (threadIdx.x < 2) reinterpret_cast<uint64_t*>((char*)dst + OFF_PROGRESS)[threadIdx.x] = 0;
   }
 }
 
@@ -253,6 +257,7 @@ static Prob make_prob(const pgp_problem_t* pr, const pgp_kernel_t* k, const pgp_
   p.vec = (double*)(ws + L.vec);
   p.red = (double*)(ws + OFF_RED);
   p.info = (int32_t*)(ws + OFF_INFO);
+  p.progress = (uint64_t*)(ws + OFF_PROGRESS);
   p.out_weight = pr->weight;
   p.out_prec = pr->precision;
   p.out_latent = pr->latent;
@@ -271,6 +276,7 @@ static Batch single_batch(char* ws, int Np) {
   b.dprobs = (Prob* const*)(ws + OFF_ARR);
   b.count = 1;
   b.maxNp = Np;
+  b.ticket = (uint32_t*)(ws + OFF_TICKET);
   return b;
 }
 
@@ -330,11 +336,16 @@ static void fill_result_final(pgp_fit_result_t* r, const double* g4, int info) {
   r->log_det = 2.0 * g4[2];
   if (g4[3] > 0) r->status |= PGP_INFO_NEGATIVE_W;
   if (info & PGP_INFO_NOT_PD) r->status |= PGP_INFO_NOT_PD;
+  if (info & PGP_INFO_STALLED) r->status |= PGP_INFO_STALLED;
   r->objective = -g4[0] + 0.5 * g4[1] + g4[2];
-  if (r->status & (PGP_INFO_NEGATIVE_W | PGP_INFO_NOT_PD)) r->objective = NAN;
+  if (r->status & (PGP_INFO_NEGATIVE_W | PGP_INFO_NOT_PD | PGP_INFO_STALLED)) r->objective = NAN;
 }
 
-static int default_refine(int prec) { return prec == PGP_PREC_FP64 ? 0 : (prec == PGP_PREC_FP32 ? 1 : 2); }
+// FP64 refinement steps per Newton solve by default.  The Newton residual is
+// always FP64, so an FP32 factor only slows the (inexact) Newton contraction:
+// no refinement costs ~10% more iterations but half the solves per
+// iteration.  BF16 trailing updates need the refinement.
+static int default_refine(int prec) { return prec == PGP_PREC_BF16 ? 2 : 0; }
 
 // ---------------------------------------------------------------------------
 // opt-in launch timer (HIP events on the launching stream)
@@ -840,27 +851,32 @@ struct pgp_batcher {
   int cap;
   Prob** arr[pgp::BATCH_ARRAYS];
   double* gather;
+  uint32_t* ticket;
   std::vector<double> hg;
   std::vector<pgp::BItem> items;
   std::vector<pgp::BItem> pending;
   std::vector<int> hnp[pgp::BATCH_ARRAYS];
+  std::vector<int> hm[pgp::BATCH_ARRAYS];
 };
 
 namespace pgp {
 
 static int64_t batcher_table_bytes(int cap) {
-  return align256((int64_t)BATCH_ARRAYS * cap * 8) + align256((int64_t)cap * 8 * 8);
+  return align256((int64_t)BATCH_ARRAYS * cap * 8) + align256((int64_t)cap * 8 * 8) + 256;
 }
 
 static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v) {
   std::vector<Prob*> ptrs;
   ptrs.reserve(v.size());
   std::vector<int>& hnp = bt->hnp[which];
+  std::vector<int>& hm = bt->hm[which];
   hnp.clear();
+  hm.clear();
   int maxNp = 0;
   for (BItem* b : v) {
     ptrs.push_back((Prob*)(b->ws + OFF_PROB));
     hnp.push_back(b->p.Np);
+    hm.push_back(b->p.M);
     maxNp = std::max(maxNp, (int)b->p.Np);
   }
   write_ptrs(bt->arr[which], ptrs, bt->s);
@@ -869,6 +885,8 @@ static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v
   b.count = (int)v.size();
   b.maxNp = maxNp;
   b.hnp = hnp.data();
+  b.hm = hm.data();
+  b.ticket = bt->ticket;
   return b;
 }
 
@@ -897,7 +915,13 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
   char* t = (char*)table;
   for (int i = 0; i < BATCH_ARRAYS; ++i) b->arr[i] = (Prob**)(t + (int64_t)i * capacity * 8);
   b->gather = (double*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8));
+  b->ticket = (uint32_t*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
+                          align256((int64_t)capacity * 8 * 8));
   b->hg.resize((size_t)capacity * 8);
+  if (hipMemsetAsync(b->ticket, 0, 256, b->s) != hipSuccess) {
+    delete b;
+    return fail(-100, "hipMemsetAsync of the batcher table failed");
+  }
   *out = b;
   return 0;
 }

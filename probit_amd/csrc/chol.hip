@@ -3,20 +3,26 @@
 // solve of the Newton system (probit/implicit/solvers.py:24), whose matrix
 // -(I + W K) is factored here in the symmetric form B = I + W^1/2 K W^1/2.
 //
-// Schedule for every batch member at once (c0 = 64k):
+// Schedule for every batch member at once (c0 = 64k), with the block
+// columns grouped into super-steps of G columns:
 //   F    : factor diagonal block 0 (one workgroup per member);
 //   P_k  : panel  L_ik = A_ik L_kk^-T for all row blocks i > k, as an FP64
 //          MFMA product with the stored inverse of L_kk;
-//   S_k  : trailing update A_ij -= L_ik L_jk^T on lower 64x64 tiles with MFMA
-//          (FP64 / FP32 / BF16 operands with FP32 accumulation); the
-//          workgroup owning tile (k+1, k+1) then factors that diagonal block.
+//   I_k  : inner update A_ij -= L_ik L_jk^T of the later columns j of k's
+//          super-step (i >= j), 64x64 tiles on MFMA;
+//   O    : after the super-step's last column, the outer update of every
+//          lower tile right of it with rank 64 G (FP64 / FP32 operands, or
+//          BF16 operands with FP32 accumulation);
+//   F_k+1: factor and invert the next diagonal block.
 // A diagonal block is factored by one workgroup in FP64 together with its
 // inverse (blocked by 16: each 16x16 diagonal sub-block inside one wave, the
-// sub-panels and updates by the whole workgroup), so the serial pivot chain
-// runs once per block and overlaps the other trailing tiles; the FP64
-// inverses (DINV) serve the panel products and the triangular solves.
+// sub-panels and updates by the whole workgroup); the FP64 inverses (DINV)
+// serve the panel products and the triangular solves.
 #include <math.h>
 #include <stdlib.h>
+
+#include <algorithm>
+#include <type_traits>
 
 #include "mfma.hpp"
 #include "pgp_internal.hpp"
@@ -253,10 +259,11 @@ __device__ void panel_rows(T* A, int64_t ld, const double* dinv_k, int64_t c0, i
 }
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_panel(Prob* const* probs, int k) {
-  const Prob& P = *probs[blockIdx.y];
+__global__ __launch_bounds__(NTHREADS) void k_panel(Prob* const* probs, Sched sc, int k) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int64_t c0 = (int64_t)k * TILE;
-  const int64_t r0 = c0 + (int64_t)TILE * (blockIdx.x + 1);
+  const int64_t r0 = c0 + (int64_t)TILE * (bx + 1);
   if (r0 >= P.Np) return;
   panel_rows<T>((T*)P.L, P.Np, P.dinv + (size_t)k * TILE * TILE, c0, r0);
 }
@@ -315,27 +322,118 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, 
   }
 }
 
-template <typename T, int BF16, int FUSE>
-__global__ __launch_bounds__(NTHREADS) void k_syrk(Prob* const* probs, int k) {
-  const Prob& P = *probs[blockIdx.y];
-  const int64_t c0 = (int64_t)k * TILE, c1 = c0 + TILE;
-  if (c1 >= P.Np) return;
-  const int Tk = (int)((P.Np - c1) / TILE);
-  if ((int)blockIdx.x >= Tk * (Tk + 1) / 2) return;
+// Workgroups of one launch are dealt to the 8 XCDs round-robin (workgroup g
+// runs on XCD g % 8).  Renumber them so that each XCD gets one contiguous
+// run of tiles: neighbouring tiles share panel rows, which then stay in that
+// XCD's L2.
+__device__ __forceinline__ int xcd_contiguous(int g, int total) {
+  const int x = g & 7, q = total >> 3, r = total & 7;
+  return x * q + (x < r ? x : r) + (g >> 3);
+}
+
+// C(ri.., rj..) -= sum over block columns c in [kc0, kc1) of
+// A(ri.., 64c..) A(rj.., 64c..)^T for one 64x64 tile: the rank-64(kc1-kc0)
+// update of a super-step, with the next 64-wide chunk of both panels loaded
+// into registers while the current one is multiplied from LDS.  C is read
+// and written once however long the K range.
+template <typename T, int BF16>
+__device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj) {
+  constexpr int PITCH = BF16 ? 72 : TILE + 1;
+  typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
+  __shared__ __attribute__((aligned(16))) LT sa[TILE * PITCH];
+  __shared__ __attribute__((aligned(16))) LT sb[TILE * PITCH];
+  typedef typename Acc<T>::type acc_t;
+  const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
+  T ra[16], rb[16];
+  acc_t acc[2][2];
+  zero_acc<T>(acc);
+  auto fetch = [&](int c) {
+    const int64_t c0 = (int64_t)c * TILE;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = kb + 4 * q;
+      ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
+      rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
+    }
+  };
+  fetch(kc0);
+  for (int c = kc0; c < kc1; ++c) {
+    if (c > kc0) __syncthreads();  // the previous chunk's products are done with LDS
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = kb + 4 * q;
+      if constexpr (BF16) {
+        sa[m * PITCH + k] = f2bf((float)ra[q]);
+        sb[m * PITCH + k] = f2bf((float)rb[q]);
+      } else {
+        sa[k * PITCH + m] = ra[q];
+        sb[k * PITCH + m] = rb[q];
+      }
+    }
+    __syncthreads();
+    if (c + 1 < kc1) fetch(c + 1);
+    if constexpr (BF16)
+      mma64_bf16(sa, PITCH, sb, PITCH, acc);
+    else
+      mma64_kmajor<T>(sa, PITCH, sb, PITCH, acc);
+  }
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) {
+      const int col = acc_col(bn);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T* p = &A[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(rj + col) * ld];
+        *p = *p - (T)acc[bm][bn][r];
+      }
+    }
+}
+
+// Inner update of a super-step: after panel k, the block columns j of the
+// same super-step (k < j < s_end) receive A_ij -= L_ik L_jk^T for i >= j.
+template <typename T, int BF16>
+__global__ __launch_bounds__(NTHREADS) void k_syrk_inner(Prob* const* probs, Sched sc, int k,
+                                                         int s_end) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int nb = P.Np / TILE;
+  const int e = s_end < nb ? s_end : nb;
+  int j = k + 1;
+  while (j < e && bx >= nb - j) {
+    bx -= nb - j;
+    ++j;
+  }
+  if (j >= e) return;
+  const int i = j + bx;
+  const FactorOut o{};
+  syrk_tile<T, BF16, 0>((T*)P.L, P.Np, (int64_t)k * TILE, (int64_t)i * TILE, (int64_t)j * TILE,
+                        false, o);
+}
+
+// Outer update closing a super-step [s0, s_end): every lower tile (i, j),
+// s_end <= j <= i, gets A_ij -= sum_{c in [s0, s_end)} L_ic L_jc^T.
+template <typename T, int BF16>
+__global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sched sc, int s0,
+                                                         int s_end) {
+  int bx;
+  const int g = xcd_contiguous(blockIdx.x, gridDim.x);
+  const Prob& P = *probs[sched_member(sc, g, bx)];
+  const int nb = P.Np / TILE;
+  const int Tk = nb - s_end;
+  if (Tk <= 0 || bx >= Tk * (Tk + 1) / 2) return;
   int ti, tj;
-  lower_tile_c(blockIdx.x, ti, tj);
-  const bool factor = (ti == 0 && tj == 0);
-  FactorOut o{P.L, P.Np, P.dinv + (size_t)(k + 1) * TILE * TILE,
-              P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr, P.info};
-  syrk_tile<T, BF16, FUSE>((T*)P.L, P.Np, c0, c1 + (int64_t)TILE * ti, c1 + (int64_t)TILE * tj,
-                           factor, o);
+  lower_tile_c(bx, ti, tj);
+  syrk_tile_multi<T, BF16>((T*)P.L, P.Np, s0, s_end, (int64_t)(s_end + ti) * TILE,
+                           (int64_t)(s_end + tj) * TILE);
 }
 
 // Diagonal block k (k >= 1) after the trailing update of step k - 1 (separate
 // mode): one workgroup per member.
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_factor_diag(Prob* const* probs, int k) {
-  const Prob& P = *probs[blockIdx.y];
+__global__ __launch_bounds__(NTHREADS) void k_factor_diag(Prob* const* probs, Sched sc, int k) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int64_t c0 = (int64_t)k * TILE;
   if (c0 >= P.Np) return;
   __shared__ double blk[2][TILE][TILE + 1];
@@ -412,57 +510,95 @@ __global__ __launch_bounds__(NTHREADS) void k_dinv_raw(const T* A, int64_t ld, d
   invert_diag_block<T>(A, ld, (int64_t)blockIdx.x * TILE, dinv + (size_t)blockIdx.x * TILE * TILE);
 }
 
-// Diagonal factorisation fused into the trailing-update tile that produces
-// the block (1) or run as its own launch after the update (0, the default:
-// it keeps the trailing-update kernel's register and LDS footprint small).
-// PGP_FUSED_FACTOR=1 selects the fused schedule.
-static bool fused_factor() {
-  static const bool v = [] {
-    const char* e = getenv("PGP_FUSED_FACTOR");
-    return e && e[0] == '1';
+// Super-step width in 64-column blocks (PGP_SUPERSTEP, default 4): the
+// trailing matrix right of a super-step is updated once per super-step with
+// rank 64 * width, so it is read and written width times less often than
+// with rank-64 updates.
+static int superstep() {
+  static const int v = [] {
+    const char* e = getenv("PGP_SUPERSTEP");
+    const int w = e ? atoi(e) : 4;
+    return w < 1 ? 1 : (w > 16 ? 16 : w);
   }();
   return v;
 }
 
 // ---------------------------------------------------------------------------
+// Per block column k (every member at once):
+//   P_k   panel below the diagonal block;
+//   I_k   inner update of the remaining columns of k's super-step;
+//   O     (last column of a super-step) outer update of everything right of it;
+//   F_k+1 factor (and invert) the next diagonal block.
 template <typename T, int BF16>
 static void potrf_steps(const Batch& b, hipStream_t s) {
   const int nb = b.maxNp / TILE;
+  const int G = superstep();
   const bool tsy = timer_on(TC_SYRK), tpa = timer_on(TC_PANEL);
   if (tpa) timer_begin(TC_PANEL, s);
   hipLaunchKernelGGL(k_factor_first<T>, dim3(1, b.count), dim3(NTHREADS), 0, s, b.dprobs);
   if (tpa) timer_end(TC_PANEL, s, b.count * (TILE * TILE * TILE / 3.0), 0.0);
   for (int k = 0; k + 1 < nb; ++k) {
-    const int below = nb - k - 1;
-    // algorithmic work summed over members: panel n * 64^2, trailing
-    // n (n + 1) * 64 (lower triangle, rank 64), next diagonal block 64^3 / 3,
-    // with n the member's trailing size after this panel.
-    double fp = 0.0, fs = 0.0;
+    const int s0 = k / G * G, s_end = s0 + G;
+    const bool outer = (k + 1 == s_end);
+    // algorithmic work summed over members (lower triangle incl. diagonal
+    // tiles' full 64x64 products counted as n (n + 1) K for an n x n SYRK)
+    double fp = 0.0, fs = 0.0, nfd = 0.0;
     if (tsy || tpa) {
       for (int m = 0; m < b.count; ++m) {
-        const double np = b.hnp ? b.hnp[m] : b.maxNp;
-        const double n = np - (double)TILE * (k + 1);
-        if (n <= 0) continue;
-        fp += n * TILE * TILE;
-        fs += n * (n + 1) * TILE + TILE * TILE * TILE / 3.0;
+        const int nbm = b.np(m) / TILE;
+        if (nbm <= k + 1) continue;
+        fp += (double)(nbm - k - 1) * TILE * TILE * TILE;
+        nfd += 1.0;
+        if (!outer) {
+          const int e = std::min(s_end, nbm);
+          for (int j = k + 1; j < e; ++j) fs += 2.0 * (nbm - j) * TILE * TILE * TILE;
+        } else if (nbm > s_end) {
+          const double n = (double)(nbm - s_end) * TILE;
+          fs += n * (n + 1) * (double)(TILE * G);
+        }
       }
     }
     if (tpa) timer_begin(TC_PANEL, s);
-    hipLaunchKernelGGL(k_panel<T>, dim3(below, b.count), dim3(NTHREADS), 0, s, b.dprobs, k);
+    sched_launch(
+        b, [&](int m) { return std::max(0, b.np(m) / TILE - k - 1); },
+        [&](Prob* const* dp, const Sched& sc, int n) {
+          hipLaunchKernelGGL(k_panel<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k);
+        });
     if (tpa) timer_end(TC_PANEL, s, fp, 0.0);
     if (tsy) timer_begin(TC_SYRK, s);
-    if (fused_factor())
-      hipLaunchKernelGGL((k_syrk<T, BF16, 1>), dim3(below * (below + 1) / 2, b.count),
-                         dim3(NTHREADS), 0, s, b.dprobs, k);
-    else
-      hipLaunchKernelGGL((k_syrk<T, BF16, 0>), dim3(below * (below + 1) / 2, b.count),
-                         dim3(NTHREADS), 0, s, b.dprobs, k);
-    if (tsy) timer_end(TC_SYRK, s, fs, 0.0);
-    if (!fused_factor()) {
-      if (tpa) timer_begin(TC_PANEL, s);
-      hipLaunchKernelGGL(k_factor_diag<T>, dim3(1, b.count), dim3(NTHREADS), 0, s, b.dprobs, k + 1);
-      if (tpa) timer_end(TC_PANEL, s, b.count * (TILE * TILE * TILE / 3.0), 0.0);
+    if (!outer) {
+      sched_launch(
+          b,
+          [&](int m) {
+            const int nbm = b.np(m) / TILE, e = std::min(s_end, nbm);
+            int t = 0;
+            for (int j = k + 1; j < e; ++j) t += nbm - j;
+            return t;
+          },
+          [&](Prob* const* dp, const Sched& sc, int n) {
+            hipLaunchKernelGGL((k_syrk_inner<T, BF16>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, k,
+                               s_end);
+          });
+    } else {
+      sched_launch(
+          b,
+          [&](int m) {
+            const int t = std::max(0, b.np(m) / TILE - s_end);
+            return t * (t + 1) / 2;
+          },
+          [&](Prob* const* dp, const Sched& sc, int n) {
+            hipLaunchKernelGGL((k_syrk_outer<T, BF16>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, s0,
+                               s_end);
+          });
     }
+    if (tsy) timer_end(TC_SYRK, s, fs, 0.0);
+    if (tpa) timer_begin(TC_PANEL, s);
+    sched_launch(
+        b, [&](int m) { return b.np(m) / TILE > k + 1 ? 1 : 0; },
+        [&](Prob* const* dp, const Sched& sc, int n) {
+          hipLaunchKernelGGL(k_factor_diag<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k + 1);
+        });
+    if (tpa) timer_end(TC_PANEL, s, nfd * (TILE * TILE * TILE / 3.0), 0.0);
   }
 }
 

@@ -57,6 +57,11 @@ __device__ __forceinline__ TC kernel_value(int kind, int periodic, int D, double
 }
 
 // Decode a linear lower-triangular tile index t into (ti >= tj).
+static inline int lower_tiles(int np) {
+  const int t = np / TILE;
+  return t * (t + 1) / 2;
+}
+
 __device__ __forceinline__ void lower_tile(int t, int& ti, int& tj) {
   int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((i + 1) * (i + 2) / 2 <= t) ++i;
@@ -119,27 +124,30 @@ __device__ void gram_tile(int kind, int periodic, int D, double variance, const 
 }
 
 // ---- batched self Gram: K64 (Np x Np, full, zero padding) per problem ----
-__global__ __launch_bounds__(NTHREADS) void k_gram_self(Prob* const* probs) {
-  const Prob& P = *probs[blockIdx.y];
+__global__ __launch_bounds__(NTHREADS) void k_gram_self(Prob* const* probs, Sched sc) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int T = P.Np / TILE;
-  if ((int)blockIdx.x >= T * (T + 1) / 2) return;
+  if (bx >= T * (T + 1) / 2) return;
   int ti, tj;
-  lower_tile(blockIdx.x, ti, tj);
+  lower_tile(bx, ti, tj);
   gram_tile<double, double>(P.kern_kind, P.periodic, P.D, P.variance, P.inv_ls2, P.pi_over_per,
                             P.X, P.N, P.X, P.N, P.K, P.Np, ti, tj, ti != tj, P.Np, P.Np);
 }
 
 void launch_gram_self(const Batch& b, hipStream_t s) {
-  const int T = b.maxNp / TILE;
-  dim3 grid(T * (T + 1) / 2, b.count);
   const bool on = timer_on(TC_GRAM);
   if (on) timer_begin(TC_GRAM, s);
-  hipLaunchKernelGGL(k_gram_self, grid, dim3(NTHREADS), 0, s, b.dprobs);
+  sched_launch(
+      b, [&](int m) { return lower_tiles(b.np(m)); },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        hipLaunchKernelGGL(k_gram_self, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+      });
   if (on) {
     // algorithmic bytes: the Np x Np FP64 matrix written once
     double bytes = 0.0;
     for (int m = 0; m < b.count; ++m) {
-      const double np = b.hnp ? b.hnp[m] : b.maxNp;
+      const double np = b.np(m);
       bytes += np * np * 8.0;
     }
     timer_end(TC_GRAM, s, 0.0, bytes);
@@ -148,12 +156,13 @@ void launch_gram_self(const Batch& b, hipStream_t s) {
 
 // ---- B = I + S K S (Laplace) or sigma^2 I + K (VB) into the factor storage ----
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, int vb) {
-  const Prob& P = *probs[blockIdx.y];
+__global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, Sched sc, int vb) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int Tn = P.Np / TILE;
-  if ((int)blockIdx.x >= Tn * (Tn + 1) / 2) return;
+  if (bx >= Tn * (Tn + 1) / 2) return;
   int ti, tj;
-  lower_tile(blockIdx.x, ti, tj);
+  lower_tile(bx, ti, tj);
   const int il = threadIdx.x & 63;
   const int jg = threadIdx.x >> 6;
   const int i = ti * TILE + il;
@@ -176,28 +185,30 @@ __global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, int vb)
 }
 
 void launch_form_B(const Batch& b, int vb, hipStream_t s) {
-  const int T = b.maxNp / TILE;
-  dim3 grid(T * (T + 1) / 2, b.count);
-  // all members of a batch share one storage precision; read it on the host
-  // side through the caller (prec passed via vb high bits)
+  // all members of a batch share one storage precision (prec in vb's high bits)
   const int prec = vb >> 8;
   const int isvb = vb & 0xff;
-  if (prec == PGP_PREC_FP64)
-    hipLaunchKernelGGL(k_form_B<double>, grid, dim3(NTHREADS), 0, s, b.dprobs, isvb);
-  else
-    hipLaunchKernelGGL(k_form_B<float>, grid, dim3(NTHREADS), 0, s, b.dprobs, isvb);
+  sched_launch(
+      b, [&](int m) { return lower_tiles(b.np(m)); },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        if (prec == PGP_PREC_FP64)
+          hipLaunchKernelGGL(k_form_B<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, isvb);
+        else
+          hipLaunchKernelGGL(k_form_B<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, isvb);
+      });
 }
 
 // ---- cross Gram for predict: S = s .* Kfs (factor storage), partial means ----
 // grid: x = (Np/64) * (Mp/64) tiles, y = batch member.  S (Np x Mp, ld Np)
 // and mupart ((Np/64) x Mp doubles) live in the member's predict workspace.
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_cross_prep(Prob* const* probs) {
-  const Prob& P = *probs[blockIdx.y];
+__global__ __launch_bounds__(NTHREADS) void k_cross_prep(Prob* const* probs, Sched sc) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int M = P.M;
   const int nrt = P.Np / TILE, nct = P.Mp / TILE;
-  if ((int)blockIdx.x >= nrt * nct) return;
-  const int ti = blockIdx.x % nrt, tj = blockIdx.x / nrt;
+  if (bx >= nrt * nct) return;
+  const int ti = bx % nrt, tj = bx / nrt;
   __shared__ double xs[TILE * PGP_MAX_DIM];
   __shared__ double zs[TILE * PGP_MAX_DIM];
   __shared__ double part[4][16];
@@ -234,12 +245,18 @@ __global__ __launch_bounds__(NTHREADS) void k_cross_prep(Prob* const* probs) {
 }
 
 void launch_cross_prep(const Batch& b, int prec, int maxM, hipStream_t s) {
-  const int Mp = (maxM + TILE - 1) / TILE * TILE;
-  dim3 grid((b.maxNp / TILE) * (Mp / TILE), b.count);
-  if (prec == PGP_PREC_FP64)
-    hipLaunchKernelGGL(k_cross_prep<double>, grid, dim3(NTHREADS), 0, s, b.dprobs);
-  else
-    hipLaunchKernelGGL(k_cross_prep<float>, grid, dim3(NTHREADS), 0, s, b.dprobs);
+  sched_launch(
+      b,
+      [&](int m) {
+        const int mt = (b.mcount(m, maxM) + TILE - 1) / TILE;
+        return (b.np(m) / TILE) * mt;
+      },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        if (prec == PGP_PREC_FP64)
+          hipLaunchKernelGGL(k_cross_prep<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+        else
+          hipLaunchKernelGGL(k_cross_prep<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+      });
 }
 
 // ---- raw Gram (kernel-level ABI) ----

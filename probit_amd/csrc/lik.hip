@@ -177,9 +177,11 @@ __device__ __forceinline__ LikParams likparams_from_prob(const Prob& P) {
 }
 
 // ---- Newton step: F -> G, W, S, LL, R = G - Z ----
-__global__ __launch_bounds__(NTHREADS) void k_lik_newton(Prob* const* probs, int final_order) {
-  const Prob& P = *probs[blockIdx.y];
-  const int i = blockIdx.x * NTHREADS + threadIdx.x;
+__global__ __launch_bounds__(NTHREADS) void k_lik_newton(Prob* const* probs, Sched sc,
+                                                         int final_order) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int i = bx * NTHREADS + threadIdx.x;
   if (i >= P.Np) return;
   const LikParams lp = likparams_from_prob(P);
   double* F = vslot(P, V_F);
@@ -201,14 +203,18 @@ __global__ __launch_bounds__(NTHREADS) void k_lik_newton(Prob* const* probs, int
   if (final_order >= 3) vslot(P, V_D3)[i] = d3;
 }
 
-void launch_lik_newton(const Batch& b, hipStream_t s) {
-  dim3 grid((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
-  hipLaunchKernelGGL(k_lik_newton, grid, dim3(NTHREADS), 0, s, b.dprobs, 0);
+static void lik_launch(const Batch& b, int order, hipStream_t s) {
+  sched_launch(
+      b, [&](int m) { return (b.np(m) + NTHREADS - 1) / NTHREADS; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        hipLaunchKernelGGL(k_lik_newton, dim3(n), dim3(NTHREADS), 0, s, dp, sc, order);
+      });
 }
 
+void launch_lik_newton(const Batch& b, hipStream_t s) { lik_launch(b, 0, s); }
+
 void launch_lik_final(const Batch& b, int order3, hipStream_t s) {
-  dim3 grid((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
-  hipLaunchKernelGGL(k_lik_newton, grid, dim3(NTHREADS), 0, s, b.dprobs, order3 ? 3 : 2);
+  lik_launch(b, order3 ? 3 : 2, s);
 }
 
 // ---- raw elementwise evaluation (kernel-level ABI) ----

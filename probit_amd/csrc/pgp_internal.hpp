@@ -82,6 +82,8 @@ struct Prob {
   double* pvar;
   // gradient / VB objective
   double* C;  // Np x Np FP64 (C = W^1/2 B^-1 W^1/2, or (sigma^2 I + K)^-1)
+  // single-launch triangular solves: (epoch << 32) | blocks published
+  uint64_t* progress;
 };
 
 __host__ __device__ inline double* vslot(const Prob& p, int v) {
@@ -92,8 +94,55 @@ struct Batch {
   Prob* const* dprobs;  // device array of device pointers to descriptors
   int count;            // batch members in this launch
   int maxNp;            // max padded size among members
-  const int* hnp = nullptr;  // host copy of each member's Np (for launch accounting)
+  const int* hnp = nullptr;  // host copy of each member's Np (NULL: all maxNp)
+  const int* hm = nullptr;   // host copy of each member's test-point count M
+  uint32_t* ticket = nullptr;  // device counter handing out workgroup order (zero between launches)
+  int np(int m) const { return hnp ? hnp[m] : maxNp; }
+  int mcount(int m, int dflt) const { return hm ? hm[m] : dflt; }
 };
+
+// Launch schedule of a batched kernel: the workgroups of one launch are
+// listed member after member (pre[m] = first workgroup of member m), so a
+// launch holds exactly the workgroups that have work, whatever the spread of
+// member sizes.  Passed by value as a kernel argument.
+constexpr int MAXB = 256;  // members per launch; larger batches are split
+struct Sched {
+  int32_t count;
+  int32_t pre[MAXB + 1];
+};
+
+// Member of workgroup `bid` (the largest m with pre[m] <= bid) and the
+// workgroup's index within that member.  Uniform: scalar loads only.
+__device__ __forceinline__ int sched_member(const Sched& sc, int bid, int& local) {
+  int lo = 0, hi = sc.count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sc.pre[mid] <= bid)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  local = bid - sc.pre[lo];
+  return lo;
+}
+
+// Build the schedule(s) of a launch: units(m) = workgroups member m needs;
+// go(dprobs, sched, total) launches one chunk of at most MAXB members.
+template <class Units, class Go>
+inline void sched_launch(const Batch& b, Units units, Go go) {
+  for (int off = 0; off < b.count; off += MAXB) {
+    const int n = b.count - off < MAXB ? b.count - off : MAXB;
+    Sched sc;
+    sc.count = n;
+    int t = 0;
+    for (int m = 0; m < n; ++m) {
+      sc.pre[m] = t;
+      t += units(off + m);
+    }
+    sc.pre[n] = t;
+    if (t > 0) go(b.dprobs + off, sc, t);
+  }
+}
 
 // Opt-in launch timer: HIP events around the launches of one kernel class
 // on the launching stream, with the algorithmic flops each launch performs.

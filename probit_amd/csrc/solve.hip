@@ -11,6 +11,11 @@
 // applied as a product with its precomputed FP64 inverse (DINV), so no step
 // has a serial substitution chain; the off-diagonal update is a GEMV / MFMA
 // GEMM.  Vectors are FP64 throughout; TRSM runs in the factor's storage type.
+#include <stdlib.h>
+
+#include <algorithm>
+#include <atomic>
+
 #include "mfma.hpp"
 #include "pgp_internal.hpp"
 
@@ -82,12 +87,12 @@ __device__ void trsv_bwd_step(const T* L, int64_t ld, const double* dinv, int k,
 }
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_trsv(Prob* const* probs, int k, int trans, int in,
-                                                   int out) {
-  const Prob& P = *probs[blockIdx.y];
+__global__ __launch_bounds__(NTHREADS) void k_trsv(Prob* const* probs, Sched sc, int k, int trans,
+                                                   int in, int out) {
+  int w;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, w)];
   const int nb = P.Np / TILE;
   if (k >= nb) return;
-  const int w = blockIdx.x;
   double* b = vslot(P, in);
   double* x = vslot(P, out);
   if (!trans) {
@@ -99,15 +104,217 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv(Prob* const* probs, int k, in
   }
 }
 
+// ---- single-launch wavefront TRSV --------------------------------------------
+// One workgroup per 64-row block of every member, all in one launch.  In
+// sweep order (forward: blocks 0, 1, ...; transposed: from the last block
+// down), block i accumulates L_ij x_j for every block j before it as soon as
+// x_j is published, then applies DINV_i and publishes x_i.  Publication is a
+// per-member progress word ((epoch << 32) | blocks published) written with
+// release semantics after x_i, read with acquire semantics; x travels through
+// agent-scope (L2-coherent) loads and stores.  Workgroups take their block
+// from a ticket counter in start order, so a workgroup only ever waits for
+// workgroups that are already running: the wait always ends.  A wait is
+// also bounded (SPIN_LIMIT polls); if the bound is ever reached the member
+// is flagged PGP_INFO_STALLED and the sweep runs on, so every workgroup exits.
+constexpr int SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sched sc,
+                                                        uint32_t* ticket, uint32_t epoch,
+                                                        int trans, int in, int out) {
+  __shared__ double dk[TILE * TILE];  // DINV_i, column-major
+  __shared__ double part[4][TILE];
+  __shared__ double v[TILE];
+  __shared__ int s_val;
+  if (threadIdx.x == 0) {
+    const uint32_t t = atomicAdd(ticket, 1u);
+    if (t == gridDim.x - 1) atomicExch(ticket, 0u);  // every ticket taken: reset for the next launch
+    s_val = (int)t;
+  }
+  __syncthreads();
+  const int tk = __builtin_amdgcn_readfirstlane(s_val);
+  if ((unsigned)tk >= gridDim.x) return;  // counter not zero at launch: no block to take
+  int local;
+  const Prob& P = *probs[sched_member(sc, tk, local)];
+  const int nb = P.Np / TILE;
+  const int i = trans ? nb - 1 - local : local;
+  const int64_t ld = P.Np;
+  const T* L = (const T*)P.L;
+  const double* b = vslot(P, in);
+  double* x = vslot(P, out);
+  const double* D = P.dinv + (size_t)i * TILE * TILE;
+  for (int e = threadIdx.x; e < TILE * TILE; e += NTHREADS) dk[e] = D[e];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t ri = (int64_t)i * TILE;
+  double pf = 0.0;      // forward: row l, columns 16w.. of each tile
+  double pb[16];        // transposed: rows 16w + rr of block i, lane = row of block j
+#pragma unroll
+  for (int rr = 0; rr < 16; ++rr) pb[rr] = 0.0;
+  int done = 0, ready = 0;
+  while (done < local) {
+    if (done >= ready) {
+      __syncthreads();  // everyone has read the previous s_val
+      if (threadIdx.x == 0) {
+        // poll with relaxed loads (no cache maintenance per poll), then one
+        // acquire fence once progress is seen
+        int r = 0;
+        for (int spins = 0;; ++spins) {
+          const uint64_t pv = __hip_atomic_load(P.progress, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+          r = ((uint32_t)(pv >> 32) == epoch) ? (int)(uint32_t)pv : 0;
+          if (r > done) break;
+          if (spins >= SPIN_LIMIT) {
+            atomicOr(P.info, PGP_INFO_STALLED);
+            r = local;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_val = r;
+      }
+      __syncthreads();
+      ready = __builtin_amdgcn_readfirstlane(s_val);
+    }
+    const int lim = ready < local ? ready : local;
+    // four published tiles at a time: their loads are in flight together
+    for (; done + 4 <= lim; done += 4) {
+      if (!trans) {
+        double xv[4];
+        T lv[4][16];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t rj = (int64_t)(done + u) * TILE;
+          xv[u] = ld_coherent(&x[rj + 16 * w + (l & 15)]);
+#pragma unroll
+          for (int c = 0; c < 16; ++c) lv[u][c] = L[(size_t)(ri + l) + (size_t)(rj + 16 * w + c) * ld];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int c = 0; c < 16; ++c) pf += (double)lv[u][c] * __shfl(xv[u], c);
+      } else {
+        double xl[4];
+        T lv[4][16];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t rj = (int64_t)(nb - 1 - (done + u)) * TILE;
+          xl[u] = ld_coherent(&x[rj + l]);
+#pragma unroll
+          for (int rr = 0; rr < 16; ++rr)
+            lv[u][rr] = L[(size_t)(rj + l) + (size_t)(ri + 16 * w + rr) * ld];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int rr = 0; rr < 16; ++rr) pb[rr] += (double)lv[u][rr] * xl[u];
+      }
+    }
+    for (; done < lim; ++done) {
+      const int j = trans ? nb - 1 - done : done;
+      const int64_t rj = (int64_t)j * TILE;
+      if (!trans) {
+        const double xv = ld_coherent(&x[rj + 16 * w + (l & 15)]);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          const double xc = __shfl(xv, c);
+          pf += (double)L[(size_t)(ri + l) + (size_t)(rj + 16 * w + c) * ld] * xc;
+        }
+      } else {
+        const double xl = ld_coherent(&x[rj + l]);
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr)
+          pb[rr] += (double)L[(size_t)(rj + l) + (size_t)(ri + 16 * w + rr) * ld] * xl;
+      }
+    }
+  }
+  // v = b_i - sum_j (...)
+  if (!trans) {
+    part[w][l] = pf;
+  } else {
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+      double t = pb[rr];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+      if (l == 0) part[0][16 * w + rr] = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < TILE) {
+    const double acc = trans ? part[0][l] : (part[0][l] + part[1][l]) + (part[2][l] + part[3][l]);
+    v[l] = b[ri + l] - acc;
+  }
+  __syncthreads();
+  // x_i = DINV_i v (forward) or DINV_i^T v (transposed)
+  double a = 0.0;
+#pragma unroll
+  for (int q = 16 * w; q < 16 * w + 16; ++q) a += (trans ? dk[q + l * TILE] : dk[l + q * TILE]) * v[q];
+  part[w][l] = a;
+  __syncthreads();
+  if (threadIdx.x < TILE) {  // wave 0: store x_i, one release fence, then publish
+    st_coherent(&x[ri + l], (part[0][l] + part[1][l]) + (part[2][l] + part[3][l]));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (threadIdx.x == 0)
+      __hip_atomic_store(P.progress, ((uint64_t)epoch << 32) | (uint64_t)(local + 1),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+static bool wavefront_trsv() {
+  static const bool v = [] {
+    const char* e = getenv("PGP_TRSV_WAVEFRONT");
+    return !(e && e[0] == '0');
+  }();
+  return v;
+}
+
+static uint32_t next_epoch() {
+  static std::atomic<uint32_t> g{0};
+  uint32_t e = ++g;
+  if (e == 0) e = ++g;
+  return e;
+}
+
 void launch_trsv(const Batch& bt, int prec, int trans, int in, int out, hipStream_t s) {
+  if (bt.ticket && wavefront_trsv()) {
+    sched_launch(
+        bt, [&](int m) { return bt.np(m) / TILE; },
+        [&](Prob* const* dp, const Sched& sc, int n) {
+          const uint32_t ep = next_epoch();
+          if (prec == PGP_PREC_FP64)
+            hipLaunchKernelGGL(k_trsv_wave<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc,
+                               bt.ticket, ep, trans, in, out);
+          else
+            hipLaunchKernelGGL(k_trsv_wave<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc,
+                               bt.ticket, ep, trans, in, out);
+        });
+    return;
+  }
   const int nb = bt.maxNp / TILE;
   for (int step = 0; step < nb; ++step) {
     const int k = trans ? nb - 1 - step : step;
-    dim3 g(trans ? k + 1 : nb - k, bt.count);
-    if (prec == PGP_PREC_FP64)
-      hipLaunchKernelGGL(k_trsv<double>, g, dim3(NTHREADS), 0, s, bt.dprobs, k, trans, in, out);
-    else
-      hipLaunchKernelGGL(k_trsv<float>, g, dim3(NTHREADS), 0, s, bt.dprobs, k, trans, in, out);
+    sched_launch(
+        bt,
+        [&](int m) {
+          const int nbm = bt.np(m) / TILE;
+          return k < nbm ? (trans ? k + 1 : nbm - k) : 0;
+        },
+        [&](Prob* const* dp, const Sched& sc, int n) {
+          if (prec == PGP_PREC_FP64)
+            hipLaunchKernelGGL(k_trsv<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, trans, in,
+                               out);
+          else
+            hipLaunchKernelGGL(k_trsv<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, trans, in,
+                               out);
+        });
   }
 }
 
@@ -200,23 +407,37 @@ __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* 
 }
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_trsm(Prob* const* probs, int k, int lower_rhs) {
-  const Prob& P = *probs[blockIdx.z];
+__global__ __launch_bounds__(NTHREADS) void k_trsm(Prob* const* probs, Sched sc, int k,
+                                                   int lower_rhs) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int nb = P.Np / TILE;
   if (k >= nb) return;
-  const int w = blockIdx.x, ct = blockIdx.y;
-  if (w >= nb - k || ct >= P.Mp / TILE) return;
+  const int w = bx % (nb - k), ct = bx / (nb - k);
+  if (ct >= P.Mp / TILE) return;
   trsm_step<T>((const T*)P.L, P.Np, P.dinv, k, (T*)P.S, P.Np, w, ct, lower_rhs);
 }
 
 void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_t s) {
   const int nb = bt.maxNp / TILE;
   for (int k = 0; k < nb; ++k) {
-    dim3 g(nb - k, maxMp / TILE, bt.count);
-    if (prec == PGP_PREC_FP64)
-      hipLaunchKernelGGL(k_trsm<double>, g, dim3(NTHREADS), 0, s, bt.dprobs, k, lower_rhs);
-    else
-      hipLaunchKernelGGL(k_trsm<float>, g, dim3(NTHREADS), 0, s, bt.dprobs, k, lower_rhs);
+    // lower_rhs: column tiles ct > k of an identity-like right-hand side
+    // stay zero, so only ct <= k are scheduled
+    sched_launch(
+        bt,
+        [&](int m) {
+          const int nbm = bt.np(m) / TILE;
+          if (k >= nbm) return 0;
+          const int mp = bt.hm ? (bt.hm[m] + TILE - 1) / TILE * TILE : maxMp;
+          const int nct = lower_rhs ? std::min(k + 1, mp / TILE) : mp / TILE;
+          return (nbm - k) * nct;
+        },
+        [&](Prob* const* dp, const Sched& sc, int n) {
+          if (prec == PGP_PREC_FP64)
+            hipLaunchKernelGGL(k_trsm<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
+          else
+            hipLaunchKernelGGL(k_trsm<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
+        });
   }
 }
 

@@ -25,10 +25,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 // mode 2: out = R - DL - S^2 .* Kx     (Laplace residual of (I + W K) d = r, x = DL)
 // mode 3: out = X1 - sigma^2 x - Kx    (VB residual of (sigma^2 I + K) w = rhs, rhs in X1)
 // mode 4: out = x + Kx... unused
-__global__ __launch_bounds__(NTHREADS) void k_gemv_sym(Prob* const* probs, int xs, int os,
-                                                       int mode) {
-  const Prob& P = *probs[blockIdx.y];
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(NTHREADS) void k_gemv_sym(Prob* const* probs, Sched sc, int xs,
+                                                       int os, int mode) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int i = bx * 4 + (threadIdx.x >> 6);
   if (i >= P.Np) return;
   const int lane = threadIdx.x & 63;
   const double* col = (mode >= 10 ? P.C : P.K) + (size_t)i * P.Np;
@@ -68,15 +69,20 @@ __global__ __launch_bounds__(NTHREADS) void k_gemv_sym(Prob* const* probs, int x
 }
 
 void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s) {
-  dim3 g((b.maxNp + 3) / 4, b.count);
-  hipLaunchKernelGGL(k_gemv_sym, g, dim3(NTHREADS), 0, s, b.dprobs, x_slot, out_slot, mode);
+  sched_launch(
+      b, [&](int m) { return (b.np(m) + 3) / 4; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        hipLaunchKernelGGL(k_gemv_sym, dim3(n), dim3(NTHREADS), 0, s, dp, sc, x_slot, out_slot,
+                           mode);
+      });
 }
 
 
 
-__global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, int op) {
-  const Prob& P = *probs[blockIdx.y];
-  const int i = blockIdx.x * NTHREADS + threadIdx.x;
+__global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, Sched sc, int op) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int i = bx * NTHREADS + threadIdx.x;
   if (i >= P.Np) return;
   switch (op) {
     case OP_STEP:
@@ -125,14 +131,18 @@ __global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, int op) {
 }
 
 void launch_vec_op(const Batch& b, int op, hipStream_t s) {
-  dim3 g((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
-  hipLaunchKernelGGL(k_vec, g, dim3(NTHREADS), 0, s, b.dprobs, op);
+  sched_launch(
+      b, [&](int m) { return (b.np(m) + NTHREADS - 1) / NTHREADS; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        hipLaunchKernelGGL(k_vec, dim3(n), dim3(NTHREADS), 0, s, dp, sc, op);
+      });
 }
 
-__global__ __launch_bounds__(NTHREADS) void k_load_state(Prob* const* probs, const double* weight,
-                                                         const double* W) {
-  const Prob& P = *probs[blockIdx.y];
-  const int i = blockIdx.x * NTHREADS + threadIdx.x;
+__global__ __launch_bounds__(NTHREADS) void k_load_state(Prob* const* probs, Sched sc,
+                                                         const double* weight, const double* W) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int i = bx * NTHREADS + threadIdx.x;
   if (i >= P.Np) return;
   if (weight) vslot(P, V_Z)[i] = (i < P.N) ? weight[i] : 0.0;
   if (W) {
@@ -143,21 +153,29 @@ __global__ __launch_bounds__(NTHREADS) void k_load_state(Prob* const* probs, con
 }
 
 void launch_load_state(const Batch& b, const double* weight, const double* W, hipStream_t s) {
-  dim3 g((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
-  hipLaunchKernelGGL(k_load_state, g, dim3(NTHREADS), 0, s, b.dprobs, weight, W);
+  sched_launch(
+      b, [&](int m) { return (b.np(m) + NTHREADS - 1) / NTHREADS; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        hipLaunchKernelGGL(k_load_state, dim3(n), dim3(NTHREADS), 0, s, dp, sc, weight, W);
+      });
 }
 
 // Copy a vector slot between problems' slots (same problem): dst = src.
-__global__ __launch_bounds__(NTHREADS) void k_copy_slot(Prob* const* probs, int src, int dst) {
-  const Prob& P = *probs[blockIdx.y];
-  const int i = blockIdx.x * NTHREADS + threadIdx.x;
+__global__ __launch_bounds__(NTHREADS) void k_copy_slot(Prob* const* probs, Sched sc, int src,
+                                                        int dst) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int i = bx * NTHREADS + threadIdx.x;
   if (i >= P.Np) return;
   vslot(P, dst)[i] = vslot(P, src)[i];
 }
 
 void launch_copy_slot(const Batch& b, int src, int dst, hipStream_t s) {
-  dim3 g((b.maxNp + NTHREADS - 1) / NTHREADS, b.count);
-  hipLaunchKernelGGL(k_copy_slot, g, dim3(NTHREADS), 0, s, b.dprobs, src, dst);
+  sched_launch(
+      b, [&](int m) { return (b.np(m) + NTHREADS - 1) / NTHREADS; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        hipLaunchKernelGGL(k_copy_slot, dim3(n), dim3(NTHREADS), 0, s, dp, sc, src, dst);
+      });
 }
 
 // ---- fixed-order block reduction ----
@@ -222,9 +240,10 @@ void launch_reduce(const Batch& b, int what, double* gather, hipStream_t s) {
 
 // ---- predict epilogue: one wave per test point ----
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_predict_final(Prob* const* probs) {
-  const Prob& P = *probs[blockIdx.y];
-  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(NTHREADS) void k_predict_final(Prob* const* probs, Sched sc) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int j = bx * 4 + (threadIdx.x >> 6);
   if (j >= P.M) return;
   const int lane = threadIdx.x & 63;
   const T* S = (const T*)P.S + (size_t)j * P.Np;
@@ -251,11 +270,14 @@ __global__ __launch_bounds__(NTHREADS) void k_predict_final(Prob* const* probs) 
 }
 
 void launch_predict_final(const Batch& b, int prec, int maxM, hipStream_t s) {
-  dim3 g((maxM + 3) / 4, b.count);
-  if (prec == PGP_PREC_FP64)
-    hipLaunchKernelGGL(k_predict_final<double>, g, dim3(NTHREADS), 0, s, b.dprobs);
-  else
-    hipLaunchKernelGGL(k_predict_final<float>, g, dim3(NTHREADS), 0, s, b.dprobs);
+  sched_launch(
+      b, [&](int m) { return (b.mcount(m, maxM) + 3) / 4; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        if (prec == PGP_PREC_FP64)
+          hipLaunchKernelGGL(k_predict_final<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+        else
+          hipLaunchKernelGGL(k_predict_final<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+      });
 }
 
 }  // namespace pgp

@@ -19,7 +19,7 @@ def _gp(cls_name, X, y, kind, lik, tol=1e-5, **kw):
 
 @pytest.mark.parametrize("lik", ["gaussian", "probit"])
 @pytest.mark.parametrize("kind", ["EQ", "Matern12"])
-@pytest.mark.parametrize("N,D", [(37, 1), (200, 3)])
+@pytest.mark.parametrize("N,D", [(37, 1), (200, 3), (1100, 2)])
 def test_laplace_matches_reference_iterates(lik, kind, N, D):
     X, y, theta, lp, spec = small_problem(N, N, D, lik, kind)
     gp = _gp("LaplaceGP", X, y, kind, lik)
