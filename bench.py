@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--requests-per-step", type=int, default=512)
     p.add_argument("--precision", default="fp32", choices=["bf16", "fp32", "fp64"])
     p.add_argument("--capacity", type=int, default=256)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only copy a MIB-sized device buffer 3 times and exit (the known-size "
+                        "reference of the HBM traffic counters)")
     p.add_argument("--refine", type=int, default=-1,
                    help="FP64 refinement steps per Newton solve (-1: library default)")
     p.add_argument("--plain-sample", type=int, default=128)
@@ -184,6 +187,16 @@ def main():
     from probit_amd import _lib, data
     from probit_amd.serving import ServingEngine
     assert torch.cuda.is_available(), "bench.py needs the GPU"
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read_per_copy": 4 * n,
+                                               "bytes_written_per_copy": 4 * n, "copies": 3}}))
+        return
     lib = _lib.lib()
 
     R, K, W = args.requests_per_step, args.steps, args.warmup
@@ -246,11 +259,17 @@ def main():
     ms_per_launch = ms.value / max(1, n_l.value)
     achieved = fl.value / (ms.value * 1e-3) / 1e12 if ms.value > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
+    # HBM bytes per launch of the outer update, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of this workload scaled by the copy
+    # calibration (profiles/syrk_outer_traffic.json); null if not measured
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "syrk_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "syrk_outer_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(args.precision)
+            rec = json.load(f).get(args.precision)
+        if rec:
+            traffic = rec.get("hbm_bytes_per_launch")
+    algo_bytes = by.value / max(1, n_l.value)
     line = {
         "metric": "GP posteriors/sec (batched mixed-precision serving, p99 latency reported)",
         "value": round(value, 2),
@@ -292,7 +311,7 @@ def main():
         "max_rel_mean_err_vs_fp64": float(np.max(errs)),
         "mean_newton_iterations": round(float(np.mean(iters)), 2),
         "roofline": {
-            "kernel": f"k_syrk ({args.precision} trailing update)",
+            "kernel": f"k_syrk_outer ({args.precision} rank-256 trailing update of the Cholesky)",
             "bound": "mfma",
             "achieved": round(achieved, 3),
             "peak": peak,
@@ -302,6 +321,7 @@ def main():
             "launches": int(n_l.value),
             "avg_launch_us": round(1e3 * ms_per_launch, 3),
             "algo_flops_per_launch": fl.value / max(1, n_l.value),
+            "algo_bytes_per_launch": algo_bytes,
         },
         "cpu_baseline": cpu,
     }

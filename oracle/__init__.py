@@ -16,7 +16,7 @@ Pinning: every oracle function is checked in ``tests/test_oracle.py`` against
 something other than its own formula -- the reference's only known-answer test
 (``probit/test/test_implicit.py:11-22``), scikit-learn's independent GP
 regression (mean, variance, log marginal likelihood and its gradient), SciPy's
-normal CDF/PDF, sympy / torch autograd derivatives, brute-force loops on tiny
+normal CDF/PDF, torch autograd derivatives, brute-force loops on tiny
 inputs, finite differences and closed-form special cases.  The exceptions are
 listed in ``UNPINNED`` below.
 """

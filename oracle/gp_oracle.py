@@ -13,7 +13,8 @@ Conventions (shared with the product path):
     kappa = exp(-r^2/2) (EQ) or exp(-r) (Matern12).  This is mlkernels'
     ``s * EQ().stretch(l).periodic(p)`` composition (a third-party library the
     reference calls at probit/approximators.py:171-174; its semantics are
-    restated from its published documentation, see DESIGN notes in tests).
+    restated from its published documentation and pinned by brute-force and
+    closed-form checks in tests/test_oracle.py).
   * Likelihood parameters are ``(noise_std,)`` (Gaussian) or
     ``(noise_std, cutpoints)`` with cutpoints of length J+1 including the
     -inf / +inf ends (probit/utilities.py:47-53).

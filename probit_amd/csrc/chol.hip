@@ -534,52 +534,49 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
   const int nb = b.maxNp / TILE;
   const int G = superstep();
   const bool tsy = timer_on(TC_SYRK), tpa = timer_on(TC_PANEL);
+  constexpr double F3 = TILE * TILE * TILE / 3.0;  // factor of one 64x64 block
+  constexpr double G64 = 2.0 * TILE * TILE * TILE;  // one 64x64x64 product
+  // algorithmic flops of the members [off, off + n) of a launch
+  auto flops = [&](const Sched& sc, Prob* const* dp, auto per_member) {
+    const int off = (int)(dp - b.dprobs);
+    double f = 0.0;
+    for (int m = off; m < off + sc.count; ++m) f += per_member(b.np(m) / TILE);
+    return f;
+  };
   if (tpa) timer_begin(TC_PANEL, s);
   hipLaunchKernelGGL(k_factor_first<T>, dim3(1, b.count), dim3(NTHREADS), 0, s, b.dprobs);
-  if (tpa) timer_end(TC_PANEL, s, b.count * (TILE * TILE * TILE / 3.0), 0.0);
+  if (tpa) timer_end(TC_PANEL, s, b.count * F3, 0.0);
   for (int k = 0; k + 1 < nb; ++k) {
     const int s0 = k / G * G, s_end = s0 + G;
     const bool outer = (k + 1 == s_end);
-    // algorithmic work summed over members (lower triangle incl. diagonal
-    // tiles' full 64x64 products counted as n (n + 1) K for an n x n SYRK)
-    double fp = 0.0, fs = 0.0, nfd = 0.0;
-    if (tsy || tpa) {
-      for (int m = 0; m < b.count; ++m) {
-        const int nbm = b.np(m) / TILE;
-        if (nbm <= k + 1) continue;
-        fp += (double)(nbm - k - 1) * TILE * TILE * TILE;
-        nfd += 1.0;
-        if (!outer) {
-          const int e = std::min(s_end, nbm);
-          for (int j = k + 1; j < e; ++j) fs += 2.0 * (nbm - j) * TILE * TILE * TILE;
-        } else if (nbm > s_end) {
-          const double n = (double)(nbm - s_end) * TILE;
-          fs += n * (n + 1) * (double)(TILE * G);
-        }
-      }
-    }
-    if (tpa) timer_begin(TC_PANEL, s);
     sched_launch(
         b, [&](int m) { return std::max(0, b.np(m) / TILE - k - 1); },
         [&](Prob* const* dp, const Sched& sc, int n) {
+          if (tpa) timer_begin(TC_PANEL, s);
           hipLaunchKernelGGL(k_panel<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k);
+          if (tpa)
+            timer_end(TC_PANEL, s,
+                      flops(sc, dp, [&](int nbm) { return std::max(0, nbm - k - 1) * G64; }), 0.0);
         });
-    if (tpa) timer_end(TC_PANEL, s, fp, 0.0);
-    if (tsy) timer_begin(TC_SYRK, s);
     if (!outer) {
+      auto inner_tiles = [&](int nbm) {
+        int t = 0;
+        for (int j = k + 1; j < std::min(s_end, nbm); ++j) t += nbm - j;
+        return t;
+      };
       sched_launch(
-          b,
-          [&](int m) {
-            const int nbm = b.np(m) / TILE, e = std::min(s_end, nbm);
-            int t = 0;
-            for (int j = k + 1; j < e; ++j) t += nbm - j;
-            return t;
-          },
+          b, [&](int m) { return inner_tiles(b.np(m) / TILE); },
           [&](Prob* const* dp, const Sched& sc, int n) {
+            if (tpa) timer_begin(TC_PANEL, s);
             hipLaunchKernelGGL((k_syrk_inner<T, BF16>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, k,
                                s_end);
+            if (tpa)
+              timer_end(TC_PANEL, s, flops(sc, dp, [&](int nbm) { return inner_tiles(nbm) * G64; }),
+                        0.0);
           });
     } else {
+      // outer update: an n x n lower triangle (n = 64 T) with rank K = 64 G
+      // counts n (n + 1) K flops
       sched_launch(
           b,
           [&](int m) {
@@ -587,18 +584,32 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
             return t * (t + 1) / 2;
           },
           [&](Prob* const* dp, const Sched& sc, int n) {
+            if (tsy) timer_begin(TC_SYRK, s);
             hipLaunchKernelGGL((k_syrk_outer<T, BF16>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, s0,
                                s_end);
+            if (tsy) {
+              // bytes: every lower tile read and written once, each trailing
+              // row block's G panel blocks read once
+              const double es = sizeof(T);
+              timer_end(TC_SYRK, s, flops(sc, dp,
+                                          [&](int nbm) {
+                                            const double nn = (double)std::max(0, nbm - s_end) * TILE;
+                                            return nn * (nn + 1) * (double)(TILE * G);
+                                          }),
+                        flops(sc, dp, [&](int nbm) {
+                          const double t = std::max(0, nbm - s_end);
+                          return (t * (t + 1) * TILE * TILE + t * TILE * TILE * G) * es;
+                        }));
+            }
           });
     }
-    if (tsy) timer_end(TC_SYRK, s, fs, 0.0);
-    if (tpa) timer_begin(TC_PANEL, s);
     sched_launch(
-        b, [&](int m) { return b.np(m) / TILE > k + 1 ? 1 : 0; },
-        [&](Prob* const* dp, const Sched& sc, int n) {
-          hipLaunchKernelGGL(k_factor_diag<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k + 1);
-        });
-    if (tpa) timer_end(TC_PANEL, s, nfd * (TILE * TILE * TILE / 3.0), 0.0);
+          b, [&](int m) { return b.np(m) / TILE > k + 1 ? 1 : 0; },
+          [&](Prob* const* dp, const Sched& sc, int n) {
+            if (tpa) timer_begin(TC_PANEL, s);
+            hipLaunchKernelGGL(k_factor_diag<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k + 1);
+            if (tpa) timer_end(TC_PANEL, s, n * F3, 0.0);
+          });
   }
 }
 

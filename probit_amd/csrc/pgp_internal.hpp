@@ -146,6 +146,11 @@ inline void sched_launch(const Batch& b, Units units, Go go) {
 
 // Opt-in launch timer: HIP events around the launches of one kernel class
 // on the launching stream, with the algorithmic flops each launch performs.
+//   TC_SYRK  outer (rank 64 G) trailing update of the Cholesky
+//   TC_PANEL per-column Cholesky steps: diagonal factor, panel, inner update
+//   TC_TRSV  single-launch triangular solves
+//   TC_GRAM  Gram assembly
+//   TC_TRSM  multi-right-hand-side solves
 enum TimerClass { TC_SYRK = 0, TC_PANEL, TC_TRSV, TC_GRAM, TC_TRSM, TC_COUNT };
 void timer_begin(int cls, hipStream_t s);
 void timer_end(int cls, hipStream_t s, double algo_flops, double algo_bytes);

@@ -285,16 +285,31 @@ static uint32_t next_epoch() {
 
 void launch_trsv(const Batch& bt, int prec, int trans, int in, int out, hipStream_t s) {
   if (bt.ticket && wavefront_trsv()) {
+    const bool on = timer_on(TC_TRSV);
+    const double esz = prec == PGP_PREC_FP64 ? 8.0 : 4.0;
     sched_launch(
         bt, [&](int m) { return bt.np(m) / TILE; },
         [&](Prob* const* dp, const Sched& sc, int n) {
           const uint32_t ep = next_epoch();
+          // algorithmic work: Np^2 flops; bytes: the lower triangle of L and
+          // the diagonal-block inverses, read once
+          double fl = 0.0, by = 0.0;
+          if (on) {
+            const int off = (int)(dp - bt.dprobs);
+            for (int m = off; m < off + sc.count; ++m) {
+              const double np = bt.np(m);
+              fl += np * np;
+              by += 0.5 * np * (np + TILE) * esz + np * TILE * 8.0;
+            }
+            timer_begin(TC_TRSV, s);
+          }
           if (prec == PGP_PREC_FP64)
             hipLaunchKernelGGL(k_trsv_wave<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc,
                                bt.ticket, ep, trans, in, out);
           else
             hipLaunchKernelGGL(k_trsv_wave<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc,
                                bt.ticket, ep, trans, in, out);
+          if (on) timer_end(TC_TRSV, s, fl, by);
         });
     return;
   }
@@ -420,23 +435,39 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm(Prob* const* probs, Sched sc,
 
 void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_t s) {
   const int nb = bt.maxNp / TILE;
+  const bool on = timer_on(TC_TRSM);
   for (int k = 0; k < nb; ++k) {
     // lower_rhs: column tiles ct > k of an identity-like right-hand side
     // stay zero, so only ct <= k are scheduled
+    auto nct_of = [&](int m) {
+      const int mp = bt.hm ? (bt.hm[m] + TILE - 1) / TILE * TILE : maxMp;
+      return lower_rhs ? std::min(k + 1, mp / TILE) : mp / TILE;
+    };
     sched_launch(
         bt,
         [&](int m) {
           const int nbm = bt.np(m) / TILE;
-          if (k >= nbm) return 0;
-          const int mp = bt.hm ? (bt.hm[m] + TILE - 1) / TILE * TILE : maxMp;
-          const int nct = lower_rhs ? std::min(k + 1, mp / TILE) : mp / TILE;
-          return (nbm - k) * nct;
+          return k < nbm ? (nbm - k) * nct_of(m) : 0;
         },
         [&](Prob* const* dp, const Sched& sc, int n) {
+          if (on) timer_begin(TC_TRSM, s);
           if (prec == PGP_PREC_FP64)
             hipLaunchKernelGGL(k_trsm<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
           else
             hipLaunchKernelGGL(k_trsm<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
+          if (on) {
+            // step k of an N^2 M-flop solve: the 64-row triangle (64^2 per
+            // column) and the update of the rows below (2 * 64 per entry)
+            double fl = 0.0;
+            const int off = (int)(dp - bt.dprobs);
+            for (int m = off; m < off + sc.count; ++m) {
+              const int nbm = bt.np(m) / TILE;
+              if (k >= nbm) continue;
+              const double cols = (double)nct_of(m) * TILE;
+              fl += cols * (TILE * TILE + 2.0 * TILE * TILE * (nbm - k - 1));
+            }
+            timer_end(TC_TRSM, s, fl, 0.0);
+          }
         });
   }
 }

@@ -3,7 +3,7 @@
 * The reference's only known-answer test (probit/test/test_implicit.py:11-22).
 * scikit-learn's independent GP regression (posterior mean/variance, log
   marginal likelihood and its gradient) for the Gaussian Laplace path.
-* SciPy's normal CDF/PDF, sympy and torch-autograd derivatives for the
+* SciPy's normal CDF/PDF and torch-autograd derivatives for the
   likelihoods; brute-force loops for the Gram matrices; finite differences
   and closed-form special cases for Newton / objective / gradient / VB.
 """
