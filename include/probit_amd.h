@@ -54,8 +54,11 @@ enum {
   PGP_INFO_NOT_PD = 1,       /* a pivot of the factor was <= 0 or NaN */
   PGP_INFO_NEGATIVE_W = 2,   /* some precision W_i < 0 (B-form undefined) */
   PGP_INFO_MAXITER = 4,      /* fixed-point loop hit maxiter before tol */
-  PGP_INFO_STALLED = 8       /* a solve's bounded wait for another workgroup
+  PGP_INFO_STALLED = 8,      /* a solve's bounded wait for another workgroup
                                 ran out (results invalid) */
+  PGP_INFO_ESCALATED = 16    /* BF16 trailing updates lost positive
+                                definiteness; the fit continued with FP32
+                                updates (results valid) */
 };
 
 /* Kernel k(x, x') = variance * kappa(r^2),

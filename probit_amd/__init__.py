@@ -9,6 +9,7 @@ HIP kernels behind a C ABI, ``include/probit_amd.h``); build it with
 
 from .approximators import Approximator, LaplaceGP, VBGP
 from .kernels import EQ, Kernel, Matern12
+from .optimize import minimise_l_bfgs_b
 from .utilities import (BOUNDS, CutpointValueError, InvalidKernel, check_cutpoints, check_data,
                         grad_log_probit_likelihood, h, hessian_log_probit_likelihood,
                         log_gaussian_likelihood, log_probit_likelihood,
@@ -19,4 +20,5 @@ __all__ = [
     "CutpointValueError", "InvalidKernel", "check_cutpoints", "check_data",
     "grad_log_probit_likelihood", "hessian_log_probit_likelihood", "h",
     "log_gaussian_likelihood", "log_probit_likelihood", "probit_predictive_distributions",
+    "minimise_l_bfgs_b",
 ]

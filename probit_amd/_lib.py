@@ -30,6 +30,8 @@ PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16}
 INFO_NOT_PD = 1
 INFO_NEGATIVE_W = 2
 INFO_MAXITER = 4
+INFO_STALLED = 8
+INFO_ESCALATED = 16
 
 TIMER_SYRK = 0
 TIMER_PANEL = 1

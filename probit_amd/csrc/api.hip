@@ -553,22 +553,32 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   int it = 0;
   double step = INFINITY;
   const bool gauss = lik->kind == PGP_LIK_GAUSSIAN;
+  int cur = prec;  // BF16 escalates to FP32 (same storage) if its factor fails
   while (step > opts->tolerance && it < opts->maxiter) {
     const bool refactor = !gauss || it == 0;
-    newton_iter(b, refactor ? &b : nullptr, prec, refine, gather, s);
+    newton_iter(b, refactor ? &b : nullptr, cur, refine, gather, s);
     rc = check_launch();
     if (rc) return rc;
     double h2;
+    int32_t inf0;
     HIP_OK(hipMemcpyAsync(&h2, gather, sizeof h2, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&inf0, ws + OFF_INFO, sizeof inf0, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    if ((inf0 & PGP_INFO_NOT_PD) && cur == PGP_PREC_BF16) {
+      // the step was not taken (OP_ZUPD); redo the iteration with FP32 updates
+      cur = PGP_PREC_FP32;
+      result->status |= PGP_INFO_ESCALATED;
+      HIP_OK(hipMemsetAsync(ws + OFF_INFO, 0, sizeof(int32_t), s));
+      continue;
+    }
     step = sqrt(h2);
     ++it;
-    if (!(step == step)) break;  // NaN: stop, report through status
+    if (!(step == step) || (inf0 & PGP_INFO_NOT_PD)) break;  // stop, report through status
   }
   result->iterations = it;
   result->step_norm = step;
   if (step > opts->tolerance) result->status |= PGP_INFO_MAXITER;
-  laplace_finalize(b, gauss ? nullptr : &b, prec, gather, s);
+  laplace_finalize(b, gauss ? nullptr : &b, cur, gather, s);
   rc = check_launch();
   if (rc) return rc;
   double g4[4];

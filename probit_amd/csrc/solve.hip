@@ -360,65 +360,76 @@ int launch_trsv_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const d
 }
 
 // ---- TRSM: S <- L^-1 S, S is Np x Mp (ld lds), one launch per block row ----
-// WG (w, ct): X = DINV_k S_k[:, ct] (redundant per w); w == 0 stores X,
-// w >= 1 updates S_w[:, ct] -= L_wk X.  lower_rhs: S is lower triangular
-// (identity-like right-hand side): tiles with ct > k stay zero and are skipped.
+// Lagged schedule.  Launch k, workgroup (w, ct), row block i = k + w:
+//   S_i[:, ct] -= L_{i,k-1} X_{k-1}[:, ct]   (k >= 1; X_{k-1} = S_{k-1}, final
+//                                            since launch k - 1)
+//   w == 0 then finishes block k: S_k[:, ct] = DINV_k S_k[:, ct].
+// Every workgroup writes only its own rows and reads only rows finished in
+// earlier launches, so workgroups of one launch never race, whatever order
+// they run in.  Launch 0 holds only the w == 0 workgroups.
+// lower_rhs: S is lower triangular (identity-like right-hand side): tiles
+// with ct > k, and the updates of tile ct == k, are zero and are skipped.
 template <typename T>
 __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* S, int64_t lds,
                           int w, int ct, int lower_rhs) {
   __shared__ T As[TILE * (TILE + 1)];
   __shared__ T Bs[TILE * (TILE + 1)];
   typedef typename Acc<T>::type acc_t;
-  const int64_t c0 = (int64_t)k * TILE;
+  if (lower_rhs && (ct > k || (ct == k && w > 0))) return;
+  const int64_t r0 = (int64_t)(k + w) * TILE;
   const int64_t n0 = (int64_t)ct * TILE;
-  if (lower_rhs && ct > k) return;
-  // A = DINV_k (k-major: As[kk*pitch + m] = Dinv(m, kk), Dinv column-major ld 64)
-  load_block_kmajor<double, T>(As, TILE + 1, dinv + (size_t)k * TILE * TILE, TILE, 0, 0);
-  // B[kk][n] = S(c0 + kk, n0 + n)
-  {
-    const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
-#pragma unroll 4
-    for (int q = 0; q < 16; ++q) {
-      const int c = cb + 4 * q;
-      Bs[r * (TILE + 1) + c] = S[(size_t)(c0 + r) + (size_t)(n0 + c) * lds];
-    }
-  }
-  __syncthreads();
   acc_t acc[2][2];
   zero_acc<T>(acc);
-  mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
-  __syncthreads();
-  if (w == 0) {
+  if (k > 0) {
+    const int64_t cp = (int64_t)(k - 1) * TILE;
+    // A = L(r0.., cp..) (k-major), B[kk][n] = S(cp + kk, n0 + n)
+    load_block_kmajor<T, T>(As, TILE + 1, L, ld, r0, cp);
+    {
+      const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
+#pragma unroll 4
+      for (int q = 0; q < 16; ++q) {
+        const int c = cb + 4 * q;
+        Bs[r * (TILE + 1) + c] = S[(size_t)(cp + r) + (size_t)(n0 + c) * lds];
+      }
+    }
+    __syncthreads();
+    mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
+    __syncthreads();
+  }
+  if (w > 0) {
 #pragma unroll
     for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
       for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          S[(size_t)(c0 + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds] = acc[bm][bn][r];
+        for (int r = 0; r < 4; ++r) {
+          T* p = &S[(size_t)(r0 + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds];
+          *p = *p - acc[bm][bn][r];
+        }
     return;
   }
-  // X into Bs (k-major B operand: Bs[kk*pitch + n] = X(kk, n))
-#pragma unroll
-  for (int bm = 0; bm < 2; ++bm)
-#pragma unroll
-    for (int bn = 0; bn < 2; ++bn)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Bs[acc_row<T>(bm, r) * (TILE + 1) + acc_col(bn)] = acc[bm][bn][r];
-  const int64_t rw = c0 + (int64_t)TILE * w;
-  load_block_kmajor<T, T>(As, TILE + 1, L, ld, rw, c0);
-  __syncthreads();
-  zero_acc<T>(acc);
-  mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
+  // block k: S_k - update (rounded to T) as the k-major B operand, DINV_k as A
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        T* p = &S[(size_t)(rw + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds];
-        *p = *p - acc[bm][bn][r];
+        const int row = acc_row<T>(bm, r), col = acc_col(bn);
+        Bs[row * (TILE + 1) + col] =
+            S[(size_t)(r0 + row) + (size_t)(n0 + col) * lds] - acc[bm][bn][r];
       }
+  load_block_kmajor<double, T>(As, TILE + 1, dinv + (size_t)k * TILE * TILE, TILE, 0, 0);
+  __syncthreads();
+  zero_acc<T>(acc);
+  mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        S[(size_t)(r0 + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds] = acc[bm][bn][r];
 }
 
 template <typename T>
@@ -428,7 +439,8 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm(Prob* const* probs, Sched sc,
   const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int nb = P.Np / TILE;
   if (k >= nb) return;
-  const int w = bx % (nb - k), ct = bx / (nb - k);
+  const int rows = k == 0 ? 1 : nb - k;  // row blocks of this launch
+  const int w = bx % rows, ct = bx / rows;
   if (ct >= P.Mp / TILE) return;
   trsm_step<T>((const T*)P.L, P.Np, P.dinv, k, (T*)P.S, P.Np, w, ct, lower_rhs);
 }
@@ -447,7 +459,7 @@ void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_
         bt,
         [&](int m) {
           const int nbm = bt.np(m) / TILE;
-          return k < nbm ? (nbm - k) * nct_of(m) : 0;
+          return k < nbm ? (k == 0 ? 1 : nbm - k) * nct_of(m) : 0;
         },
         [&](Prob* const* dp, const Sched& sc, int n) {
           if (on) timer_begin(TC_TRSM, s);
@@ -456,15 +468,15 @@ void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_
           else
             hipLaunchKernelGGL(k_trsm<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
           if (on) {
-            // step k of an N^2 M-flop solve: the 64-row triangle (64^2 per
-            // column) and the update of the rows below (2 * 64 per entry)
+            // step k of an N^2 M-flop solve: the update of rows k.. by block
+            // k - 1 (2 * 64 per entry) and the 64-row triangle (64^2 per column)
             double fl = 0.0;
             const int off = (int)(dp - bt.dprobs);
             for (int m = off; m < off + sc.count; ++m) {
               const int nbm = bt.np(m) / TILE;
               if (k >= nbm) continue;
               const double cols = (double)nct_of(m) * TILE;
-              fl += cols * (TILE * TILE + 2.0 * TILE * TILE * (nbm - k - 1));
+              fl += cols * (TILE * TILE + (k > 0 ? 2.0 * TILE * TILE * (nbm - k) : 0.0));
             }
             timer_end(TC_TRSM, s, fl, 0.0);
           }
@@ -482,7 +494,7 @@ int launch_trsm_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const d
                          void* S, int64_t M, int64_t lds, int lower_rhs, hipStream_t s) {
   const int nb = (int)(N / TILE);
   for (int k = 0; k < nb; ++k) {
-    dim3 g(nb - k, (unsigned)(M / TILE));
+    dim3 g(k == 0 ? 1 : nb - k, (unsigned)(M / TILE));
     if (prec == PGP_PREC_FP64)
       hipLaunchKernelGGL(k_trsm_raw<double>, g, dim3(NTHREADS), 0, s, (const double*)L, ld, dinv,
                          k, (double*)S, lds, lower_rhs);

@@ -91,8 +91,8 @@ __global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, Sched sc, 
     case OP_IRCORR:
       vslot(P, V_DL)[i] += vslot(P, V_Q)[i] - vslot(P, V_S)[i] * vslot(P, V_U)[i];
       break;
-    case OP_ZUPD:
-      vslot(P, V_Z)[i] += vslot(P, V_DL)[i];
+    case OP_ZUPD:  // a step from a failed factor is not taken
+      if (!(P.info[0] & PGP_INFO_NOT_PD)) vslot(P, V_Z)[i] += vslot(P, V_DL)[i];
       break;
     case OP_ZERO_Z:
       vslot(P, V_Z)[i] = 0.0;

@@ -143,6 +143,24 @@ def ordinal_ard(seed=2, N=32768, M=4096, D=8, J=5, noise_variance=0.1):
                    cutpoints=cut, Xtest=Xa[N:], extra={"y_test": y[N:]})
 
 
+def evidence_gradient(seed=3, N=8192, lik="gaussian", D=4, lengthscale=0.8, variance=1.5):
+    """Evidence-gradient problem: X ~ N(0, I_D), latent from the EQ prior
+    (lengthscale 0.8, signal variance 1.5); Gaussian targets with noise std
+    0.2, or ordinal probit J=3 with noise variance 0.4 and quantile-balanced
+    cutpoints."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((N, D))
+    f = (prior_sample_exact(rng, EQ, X, lengthscale, variance, jitter=1e-8) if N <= 3000
+         else prior_sample_rff(rng, EQ, X, lengthscale, variance))
+    if lik == "gaussian":
+        y = f + 0.2 * rng.standard_normal(N)
+        return Problem(X, y, EQ, lengthscale, variance, "gaussian", 0.2)
+    g = f + math.sqrt(0.4) * rng.standard_normal(N)
+    cut = quantile_cutpoints(g, 3)
+    return Problem(X, bin_labels(g, cut), EQ, lengthscale, variance, "probit", math.sqrt(0.4),
+                   cutpoints=cut)
+
+
 @dataclass
 class Request:
     rid: int
