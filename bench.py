@@ -47,7 +47,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--requests-per-step", type=int, default=512)
     p.add_argument("--precision", default="fp32", choices=["bf16", "fp32", "fp64"])
-    p.add_argument("--capacity", type=int, default=256)
+    p.add_argument("--capacity", type=int, default=512)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-sized device buffer 3 times and exit (the known-size "
                         "reference of the HBM traffic counters)")
@@ -74,27 +74,31 @@ def upload(reqs, torch):
     return out
 
 
-def serve_window(engine, window, t_arrive=None):
-    """Serve a window to completion; returns {rid: (latency_s, Completed)}."""
+def serve_window(engine, window, t_arrive=None, chunk=128):
+    """Serve a window to completion; returns {rid: (latency_s, Completed)}.
+    Asynchronous steps: while the GPU runs a step, the host submits up to
+    `chunk` more requests (admitted at the next launch)."""
     import time as _t
     pending = list(window)
     t0 = _t.perf_counter()
     res = {}
     while pending or engine.in_flight:
-        still = []
+        launched = engine.launch()
+        still, n = [], 0
         for r in pending:
-            p = r["p"]
-            if t_arrive is not None and t_arrive[r["rid"]] > _t.perf_counter():
+            if n >= chunk or (t_arrive is not None and t_arrive[r["rid"]] > _t.perf_counter()):
                 still.append(r)
                 continue
-            ok = engine.submit(r["rid"], r["X"], r["y"], p.kind, p.lengthscale, p.variance, p.lik,
-                               p.noise_std, p.cutpoints, r["Xt"], p.period)
-            if not ok:
+            p = r["p"]
+            if engine.submit(r["rid"], r["X"], r["y"], p.kind, p.lengthscale, p.variance, p.lik,
+                             p.noise_std, p.cutpoints, r["Xt"], p.period):
+                n += 1
+            else:
                 still.append(r)
         pending = still
-        if engine.in_flight == 0:
+        if not launched:
             continue
-        for c in engine.step():
+        for c in engine.finish():
             t_done = _t.perf_counter()
             start = t0 if t_arrive is None else t_arrive[c.tag]
             res[c.tag] = (t_done - start, c)

@@ -297,6 +297,17 @@ int pgp_batcher_submit(pgp_batcher_t* batcher, pgp_batch_item_t* item,
                        int64_t tag);
 int pgp_batcher_step(pgp_batcher_t* batcher, int64_t* done_tags,
                      int32_t max_done, int32_t* n_done);
+/* Asynchronous form of pgp_batcher_step, so that the host can prepare and
+ * submit requests while the GPU works: launch() admits queued items and
+ * enqueues one step (the finalisation and predict of the items that
+ * converged in the previous step, one Newton iteration for all others) and
+ * returns at once; finish() waits for that step and returns the tags of the
+ * items it completed.  An item is reported one step after it converges;
+ * its results are the same as with pgp_batcher_step.  Do not mix the two
+ * forms while a launch is pending. */
+int pgp_batcher_launch(pgp_batcher_t* batcher);
+int pgp_batcher_finish(pgp_batcher_t* batcher, int64_t* done_tags,
+                       int32_t max_done, int32_t* n_done);
 int32_t pgp_batcher_in_flight(pgp_batcher_t* batcher);
 int pgp_batcher_destroy(pgp_batcher_t* batcher);
 

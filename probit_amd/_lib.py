@@ -153,6 +153,8 @@ _SIGNATURES = {
                                      C.POINTER(C.c_void_p)]),
     "pgp_batcher_submit": (C.c_int, [_P, C.POINTER(BatchItemT), _I64]),
     "pgp_batcher_step": (C.c_int, [_P, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int32)]),
+    "pgp_batcher_launch": (C.c_int, [_P]),
+    "pgp_batcher_finish": (C.c_int, [_P, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int32)]),
     "pgp_batcher_in_flight": (_I32, [_P]),
     "pgp_batcher_destroy": (C.c_int, [_P]),
 }

@@ -846,10 +846,11 @@ struct BItem {
   int64_t tag;
   char* ws;
   Prob p;
-  int phase;  // 0 = new, 1 = Newton
+  int phase;  // 0 = new, 1 = Newton, 2 = converged (finalise next), 3 = finalising
   int iter;
   double step;
   bool gauss;
+  bool done;
 };
 
 }  // namespace pgp
@@ -860,8 +861,15 @@ struct pgp_batcher {
   int refine;
   int cap;
   Prob** arr[pgp::BATCH_ARRAYS];
-  double* gather;
+  double* gather;      // per-member step norms (device)
+  double* gather_fin;  // 8 doubles per finalised member (device)
   uint32_t* ticket;
+  // asynchronous launch / finish
+  double* hstep = nullptr;  // pinned copies of gather / gather_fin
+  double* hfin = nullptr;
+  hipEvent_t ev = nullptr;
+  bool launched = false;
+  std::vector<pgp::BItem*> act_list, fin_list;
   std::vector<double> hg;
   std::vector<pgp::BItem> items;
   std::vector<pgp::BItem> pending;
@@ -872,8 +880,9 @@ struct pgp_batcher {
 namespace pgp {
 
 static int64_t batcher_table_bytes(int cap) {
-  return align256((int64_t)BATCH_ARRAYS * cap * 8) + align256((int64_t)cap * 8 * 8) + 256;
+  return align256((int64_t)BATCH_ARRAYS * cap * 8) + 2 * align256((int64_t)cap * 8 * 8) + 256;
 }
+
 
 static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v) {
   std::vector<Prob*> ptrs;
@@ -898,6 +907,100 @@ static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v
   b.hm = hm.data();
   b.ticket = bt->ticket;
   return b;
+}
+
+// Admit pending requests up to capacity; new members get their descriptor,
+// Gram matrix and a zero weight.
+static void admit(pgp_batcher* bt) {
+  hipStream_t s = bt->s;
+  size_t take = 0;
+  while (take < bt->pending.size() && (int)bt->items.size() < bt->cap) {
+    bt->items.push_back(bt->pending[take]);
+    ++take;
+  }
+  bt->pending.erase(bt->pending.begin(), bt->pending.begin() + take);
+  std::vector<BItem*> fresh;
+  for (BItem& b : bt->items)
+    if (b.phase == 0) {
+      write_prob(b.ws, b.p, 1, s);
+      fresh.push_back(&b);
+    }
+  if (!fresh.empty()) {
+    const Batch bn = make_batch(bt, 0, fresh);
+    launch_gram_self(bn, s);
+    launch_vec_op(bn, OP_ZERO_Z, s);
+    for (BItem* b : fresh) b->phase = 1;
+  }
+}
+
+// Enqueue the final state (f, W, log p, B factored at W) and the predict of
+// the members in `conv`; 8 result doubles per member land in `gather`.
+static void finalize_and_predict(pgp_batcher* bt, const std::vector<BItem*>& conv,
+                                 double* gather) {
+  hipStream_t s = bt->s;
+  const int prec = bt->opts.precision;
+  std::vector<BItem*> conv_fac, pred;
+  for (BItem* b : conv)
+    if (!b->gauss) conv_fac.push_back(b);
+  const Batch bc = make_batch(bt, 3, conv);
+  const Batch bcf = make_batch(bt, 4, conv_fac);
+  laplace_finalize(bc, &bcf, prec, gather, s);
+  int maxM = 0;
+  for (BItem* b : conv) {
+    pgp_batch_item_t* it = b->item;
+    if (!it->Xtest) continue;
+    const PredLayout PL = pred_layout(b->p.Np, it->M, prec);
+    char* pw = (char*)it->predict_workspace;
+    b->p.Xt = it->Xtest;
+    b->p.M = (int32_t)it->M;
+    b->p.Mp = (int32_t)PL.Mp;
+    b->p.S = pw + PL.S;
+    b->p.mupart = (double*)(pw + PL.mupart);
+    b->p.pmean = it->mean;
+    b->p.pvar = it->var;
+    write_prob(b->ws, b->p, 0, s);
+    pred.push_back(b);
+    maxM = std::max(maxM, (int)it->M);
+  }
+  if (!pred.empty()) {
+    const Batch bp = make_batch(bt, 5, pred);
+    const int maxMp = (int)round_up(maxM, TILE);
+    launch_cross_prep(bp, prec, maxM, s);
+    launch_trsm(bp, prec, maxMp, 0, s);
+    launch_predict_final(bp, prec, maxM, s);
+  }
+}
+
+// Results of finalised members from their 8 gathered doubles each.
+static void report(pgp_batcher* bt, const std::vector<BItem*>& conv, const double* h,
+                   int64_t* done_tags, int32_t max_done, int32_t* n_done) {
+  for (size_t i = 0; i < conv.size(); ++i) {
+    BItem* b = conv[i];
+    pgp_fit_result_t* r = &b->item->result;
+    const int32_t esc = r->status & PGP_INFO_ESCALATED;
+    memset(r, 0, sizeof *r);
+    r->status = esc;
+    r->iterations = b->iter;
+    r->step_norm = b->step;
+    if (b->step > bt->opts.tolerance) r->status |= PGP_INFO_MAXITER;
+    fill_result_final(r, &h[8 * i], (int)h[8 * i + 4]);
+    if (done_tags && *n_done < max_done) done_tags[*n_done] = b->tag;
+    ++*n_done;
+    b->done = true;
+  }
+}
+
+// Drop finished members (stable order).
+static void drop_done(pgp_batcher* bt) {
+  std::vector<BItem> keep;
+  keep.reserve(bt->items.size());
+  for (BItem& b : bt->items)
+    if (!b.done) keep.push_back(b);
+  bt->items.swap(keep);
+}
+
+static bool converged(const pgp_batcher* bt, const BItem* b) {
+  return b->step <= bt->opts.tolerance || b->iter >= bt->opts.maxiter || !(b->step == b->step);
 }
 
 }  // namespace pgp
@@ -925,12 +1028,17 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
   char* t = (char*)table;
   for (int i = 0; i < BATCH_ARRAYS; ++i) b->arr[i] = (Prob**)(t + (int64_t)i * capacity * 8);
   b->gather = (double*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8));
+  b->gather_fin = (double*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
+                            align256((int64_t)capacity * 8 * 8));
   b->ticket = (uint32_t*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
-                          align256((int64_t)capacity * 8 * 8));
+                          2 * align256((int64_t)capacity * 8 * 8));
   b->hg.resize((size_t)capacity * 8);
-  if (hipMemsetAsync(b->ticket, 0, 256, b->s) != hipSuccess) {
-    delete b;
-    return fail(-100, "hipMemsetAsync of the batcher table failed");
+  if (hipMemsetAsync(b->ticket, 0, 256, b->s) != hipSuccess ||
+      hipHostMalloc((void**)&b->hstep, (size_t)capacity * 8, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&b->hfin, (size_t)capacity * 64, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess) {
+    pgp_batcher_destroy(b);
+    return fail(-100, "batcher set-up (table clear, pinned buffers, event) failed");
   }
   *out = b;
   return 0;
@@ -959,6 +1067,7 @@ int pgp_batcher_submit(pgp_batcher_t* bt, pgp_batch_item_t* item, int64_t tag) {
   b.p = make_prob(&item->problem, &item->kernel, &item->lik, prec, b.ws);
   b.phase = 0;
   b.iter = 0;
+  b.done = false;
   b.step = INFINITY;
   b.gauss = item->lik.kind == PGP_LIK_GAUSSIAN;
   memset(&item->result, 0, sizeof item->result);
@@ -974,27 +1083,10 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   if (!bt) return fail(-1, "batcher is NULL");
   if (!n_done) return fail(-4, "n_done is NULL");
   *n_done = 0;
+  if (bt->launched) return fail(-1, "pgp_batcher_step: an asynchronous launch is pending");
   hipStream_t s = bt->s;
   const int prec = bt->opts.precision;
-  // admit pending requests up to capacity
-  std::vector<BItem*> fresh;
-  size_t take = 0;
-  while (take < bt->pending.size() && (int)bt->items.size() < bt->cap) {
-    bt->items.push_back(bt->pending[take]);
-    ++take;
-  }
-  bt->pending.erase(bt->pending.begin(), bt->pending.begin() + take);
-  for (BItem& b : bt->items)
-    if (b.phase == 0) {
-      write_prob(b.ws, b.p, 1, s);
-      fresh.push_back(&b);
-    }
-  if (!fresh.empty()) {
-    const Batch bn = make_batch(bt, 0, fresh);
-    launch_gram_self(bn, s);
-    launch_vec_op(bn, OP_ZERO_Z, s);
-    for (BItem* b : fresh) b->phase = 1;
-  }
+  admit(bt);
   std::vector<BItem*> active, fac;
   for (BItem& b : bt->items) {
     active.push_back(&b);
@@ -1006,75 +1098,90 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   newton_iter(ba, &bf, prec, bt->refine, bt->gather, s);
   int rc = check_launch();
   if (rc) return rc;
-  HIP_OK(hipMemcpyAsync(bt->hg.data(), bt->gather, active.size() * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(bt->hstep, bt->gather, active.size() * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
-  std::vector<BItem*> conv, conv_fac, pred;
+  std::vector<BItem*> conv;
   for (size_t i = 0; i < active.size(); ++i) {
     BItem* b = active[i];
-    b->step = sqrt(bt->hg[i]);
+    b->step = sqrt(bt->hstep[i]);
     b->iter++;
-    if (b->step <= bt->opts.tolerance || b->iter >= bt->opts.maxiter || !(b->step == b->step)) {
-      conv.push_back(b);
-      if (!b->gauss) conv_fac.push_back(b);
-    }
+    if (converged(bt, b)) conv.push_back(b);
   }
   if (conv.empty()) return 0;
-  const Batch bc = make_batch(bt, 3, conv);
-  const Batch bcf = make_batch(bt, 4, conv_fac);
-  laplace_finalize(bc, &bcf, prec, bt->gather, s);
-  int maxM = 0;
-  for (BItem* b : conv) {
-    pgp_batch_item_t* it = b->item;
-    if (!it->Xtest) continue;
-    const PredLayout PL = pred_layout(b->p.Np, it->M, prec);
-    char* pw = (char*)it->predict_workspace;
-    b->p.Xt = it->Xtest;
-    b->p.M = (int32_t)it->M;
-    b->p.Mp = (int32_t)PL.Mp;
-    b->p.S = pw + PL.S;
-    b->p.mupart = (double*)(pw + PL.mupart);
-    b->p.pmean = it->mean;
-    b->p.pvar = it->var;
-    write_prob(b->ws, b->p, 0, s);
-    pred.push_back(b);
-    maxM = std::max(maxM, (int)it->M);
-  }
-  if (!pred.empty()) {
-    const Batch bp = make_batch(bt, 5, pred);
-    const int maxMp = (int)round_up(maxM, TILE);
-    launch_cross_prep(bp, prec, maxM, s);
-    launch_trsm(bp, prec, maxMp, 0, s);
-    launch_predict_final(bp, prec, maxM, s);
-  }
+  finalize_and_predict(bt, conv, bt->gather_fin);
   rc = check_launch();
   if (rc) return rc;
-  HIP_OK(hipMemcpyAsync(bt->hg.data(), bt->gather, conv.size() * 8 * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(bt->hfin, bt->gather_fin, conv.size() * 64, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
-  for (size_t i = 0; i < conv.size(); ++i) {
-    BItem* b = conv[i];
-    pgp_fit_result_t* r = &b->item->result;
-    memset(r, 0, sizeof *r);
-    r->iterations = b->iter;
-    r->step_norm = b->step;
-    if (b->step > bt->opts.tolerance) r->status |= PGP_INFO_MAXITER;
-    fill_result_final(r, &bt->hg[8 * i], (int)bt->hg[8 * i + 4]);
-    if (done_tags && *n_done < max_done) done_tags[*n_done] = b->tag;
-    ++*n_done;
-  }
-  // drop finished members (stable order)
-  std::vector<BItem> keep;
-  keep.reserve(bt->items.size());
+  report(bt, conv, bt->hfin, done_tags, max_done, n_done);
+  drop_done(bt);
+  return 0;
+}
+
+int pgp_batcher_launch(pgp_batcher_t* bt) {
+  if (!bt) return fail(-1, "batcher is NULL");
+  if (bt->launched) return fail(-1, "pgp_batcher_launch: the previous launch is not finished");
+  hipStream_t s = bt->s;
+  const int prec = bt->opts.precision;
+  admit(bt);
+  bt->fin_list.clear();
+  bt->act_list.clear();
+  std::vector<BItem*> fac;
   for (BItem& b : bt->items) {
-    bool done = false;
-    for (BItem* c : conv)
-      if (c == &b) done = true;
-    if (!done) keep.push_back(b);
+    if (b.phase == 2) {
+      bt->fin_list.push_back(&b);
+    } else if (b.phase == 1) {
+      bt->act_list.push_back(&b);
+      if (!b.gauss || b.iter == 0) fac.push_back(&b);
+    }
   }
-  bt->items.swap(keep);
+  if (!bt->fin_list.empty()) {
+    finalize_and_predict(bt, bt->fin_list, bt->gather_fin);
+    HIP_OK(hipMemcpyAsync(bt->hfin, bt->gather_fin, bt->fin_list.size() * 64,
+                          hipMemcpyDeviceToHost, s));
+    for (BItem* b : bt->fin_list) b->phase = 3;
+  }
+  if (!bt->act_list.empty()) {
+    const Batch ba = make_batch(bt, 1, bt->act_list);
+    const Batch bf = make_batch(bt, 2, fac);
+    newton_iter(ba, &bf, prec, bt->refine, bt->gather, s);
+    HIP_OK(hipMemcpyAsync(bt->hstep, bt->gather, bt->act_list.size() * 8,
+                          hipMemcpyDeviceToHost, s));
+  }
+  int rc = check_launch();
+  if (rc) return rc;
+  HIP_OK(hipEventRecord(bt->ev, s));
+  bt->launched = true;
+  return 0;
+}
+
+int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
+                       int32_t* n_done) {
+  if (!bt) return fail(-1, "batcher is NULL");
+  if (!n_done) return fail(-4, "n_done is NULL");
+  *n_done = 0;
+  if (!bt->launched) return 0;
+  HIP_OK(hipEventSynchronize(bt->ev));
+  bt->launched = false;
+  report(bt, bt->fin_list, bt->hfin, done_tags, max_done, n_done);
+  for (size_t i = 0; i < bt->act_list.size(); ++i) {
+    BItem* b = bt->act_list[i];
+    b->step = sqrt(bt->hstep[i]);
+    b->iter++;
+    if (converged(bt, b)) b->phase = 2;
+  }
+  bt->fin_list.clear();
+  bt->act_list.clear();
+  drop_done(bt);
   return 0;
 }
 
 int pgp_batcher_destroy(pgp_batcher_t* bt) {
+  if (!bt) return 0;
+  if (bt->launched) (void)hipEventSynchronize(bt->ev);
+  if (bt->ev) (void)hipEventDestroy(bt->ev);
+  if (bt->hstep) (void)hipHostFree(bt->hstep);
+  if (bt->hfin) (void)hipHostFree(bt->hfin);
   delete bt;
   return 0;
 }

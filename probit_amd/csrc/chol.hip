@@ -336,12 +336,18 @@ __device__ __forceinline__ int xcd_contiguous(int g, int total) {
 // update of a super-step, with the next 64-wide chunk of both panels loaded
 // into registers while the current one is multiplied from LDS.  C is read
 // and written once however long the K range.
-template <typename T, int BF16>
-__device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj) {
+// FUSE: the workgroup given factor = true (the tile that becomes the next
+// diagonal block) keeps its updated tile in LDS and factors it at once.
+template <typename T, int BF16, int FUSE>
+__device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj,
+                                bool factor, const FactorOut& o) {
   constexpr int PITCH = BF16 ? 72 : TILE + 1;
   typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
-  __shared__ __attribute__((aligned(16))) LT sa[TILE * PITCH];
-  __shared__ __attribute__((aligned(16))) LT sb[TILE * PITCH];
+  constexpr int OPER = 2 * TILE * PITCH * (int)sizeof(LT);
+  constexpr int BLK = FUSE ? 2 * TILE * (TILE + 1) * 8 : 0;
+  __shared__ __attribute__((aligned(16))) char arena[OPER > BLK ? OPER : BLK];
+  LT* sa = (LT*)arena;
+  LT* sb = sa + TILE * PITCH;
   typedef typename Acc<T>::type acc_t;
   const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
   T ra[16], rb[16];
@@ -377,6 +383,9 @@ __device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, 
     else
       mma64_kmajor<T>(sa, PITCH, sb, PITCH, acc);
   }
+  if constexpr (!FUSE) factor = false;
+  double (*blk)[TILE + 1] = (double (*)[TILE + 1])arena;
+  if (factor) __syncthreads();  // operands are dead; the arena becomes the block
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -384,15 +393,26 @@ __device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, 
       const int col = acc_col(bn);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        T* p = &A[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(rj + col) * ld];
-        *p = *p - (T)acc[bm][bn][r];
+        const int row = acc_row<T>(bm, r);
+        T* p = &A[(size_t)(ri + row) + (size_t)(rj + col) * ld];
+        const T v = *p - (T)acc[bm][bn][r];
+        if (factor)
+          blk[row][col] = (double)v;
+        else
+          *p = v;
       }
     }
+  if constexpr (FUSE) {
+    if (factor) {
+      __syncthreads();
+      factor_block_blocked<T>(blk, blk + TILE, ri, o);
+    }
+  }
 }
 
 // Inner update of a super-step: after panel k, the block columns j of the
 // same super-step (k < j < s_end) receive A_ij -= L_ik L_jk^T for i >= j.
-template <typename T, int BF16>
+template <typename T, int BF16, int FUSE>
 __global__ __launch_bounds__(NTHREADS) void k_syrk_inner(Prob* const* probs, Sched sc, int k,
                                                          int s_end) {
   int bx;
@@ -406,14 +426,17 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_inner(Prob* const* probs, Sch
   }
   if (j >= e) return;
   const int i = j + bx;
-  const FactorOut o{};
-  syrk_tile<T, BF16, 0>((T*)P.L, P.Np, (int64_t)k * TILE, (int64_t)i * TILE, (int64_t)j * TILE,
-                        false, o);
+  const int c1 = (k + 1) * TILE;
+  const FactorOut o{P.L, P.Np, P.dinv + (size_t)(k + 1) * TILE * TILE,
+                    P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr,
+                    P.info};
+  syrk_tile<T, BF16, FUSE>((T*)P.L, P.Np, (int64_t)k * TILE, (int64_t)i * TILE,
+                           (int64_t)j * TILE, FUSE && i == k + 1 && j == k + 1, o);
 }
 
 // Outer update closing a super-step [s0, s_end): every lower tile (i, j),
 // s_end <= j <= i, gets A_ij -= sum_{c in [s0, s_end)} L_ic L_jc^T.
-template <typename T, int BF16>
+template <typename T, int BF16, int FUSE>
 __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sched sc, int s0,
                                                          int s_end) {
   int bx;
@@ -424,8 +447,12 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
   if (Tk <= 0 || bx >= Tk * (Tk + 1) / 2) return;
   int ti, tj;
   lower_tile_c(bx, ti, tj);
-  syrk_tile_multi<T, BF16>((T*)P.L, P.Np, s0, s_end, (int64_t)(s_end + ti) * TILE,
-                           (int64_t)(s_end + tj) * TILE);
+  const int c1 = s_end * TILE;
+  const FactorOut o{P.L, P.Np, P.dinv + (size_t)s_end * TILE * TILE,
+                    P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr,
+                    P.info};
+  syrk_tile_multi<T, BF16, FUSE>((T*)P.L, P.Np, s0, s_end, (int64_t)(s_end + ti) * TILE,
+                                 (int64_t)(s_end + tj) * TILE, FUSE && ti == 0 && tj == 0, o);
 }
 
 // Diagonal block k (k >= 1) after the trailing update of step k - 1 (separate
@@ -510,6 +537,16 @@ __global__ __launch_bounds__(NTHREADS) void k_dinv_raw(const T* A, int64_t ld, d
   invert_diag_block<T>(A, ld, (int64_t)blockIdx.x * TILE, dinv + (size_t)blockIdx.x * TILE * TILE);
 }
 
+// Update launches of at most this many tiles factor the next diagonal block
+// inside the launch (PGP_FUSE_MAX; 0 disables).
+static int fuse_max() {
+  static const int v = [] {
+    const char* e = getenv("PGP_FUSE_MAX");
+    return e ? atoi(e) : 512;
+  }();
+  return v;
+}
+
 // Super-step width in 64-column blocks (PGP_SUPERSTEP, default 4): the
 // trailing matrix right of a super-step is updated once per super-step with
 // rank 64 * width, so it is read and written width times less often than
@@ -558,18 +595,31 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
             timer_end(TC_PANEL, s,
                       flops(sc, dp, [&](int nbm) { return std::max(0, nbm - k - 1) * G64; }), 0.0);
         });
+    // Small update launches (late columns, few members) carry the next
+    // diagonal factorisation in the workgroup of tile (k+1, k+1): one launch
+    // fewer on the critical chain.  Large ones keep the lean kernel (the
+    // factorisation's registers and LDS would lower its occupancy) and a
+    // separate factor launch.
+    bool fused = false;
     if (!outer) {
       auto inner_tiles = [&](int nbm) {
         int t = 0;
         for (int j = k + 1; j < std::min(s_end, nbm); ++j) t += nbm - j;
         return t;
       };
+      int total = 0;
+      for (int m = 0; m < b.count; ++m) total += inner_tiles(b.np(m) / TILE);
+      fused = total <= fuse_max();
       sched_launch(
           b, [&](int m) { return inner_tiles(b.np(m) / TILE); },
           [&](Prob* const* dp, const Sched& sc, int n) {
             if (tpa) timer_begin(TC_PANEL, s);
-            hipLaunchKernelGGL((k_syrk_inner<T, BF16>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, k,
-                               s_end);
+            if (fused)
+              hipLaunchKernelGGL((k_syrk_inner<T, BF16, 1>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
+                                 k, s_end);
+            else
+              hipLaunchKernelGGL((k_syrk_inner<T, BF16, 0>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
+                                 k, s_end);
             if (tpa)
               timer_end(TC_PANEL, s, flops(sc, dp, [&](int nbm) { return inner_tiles(nbm) * G64; }),
                         0.0);
@@ -577,16 +627,23 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
     } else {
       // outer update: an n x n lower triangle (n = 64 T) with rank K = 64 G
       // counts n (n + 1) K flops
+      auto outer_tiles = [&](int m) {
+        const int t = std::max(0, b.np(m) / TILE - s_end);
+        return t * (t + 1) / 2;
+      };
+      int total = 0;
+      for (int m = 0; m < b.count; ++m) total += outer_tiles(m);
+      fused = total <= fuse_max();
       sched_launch(
-          b,
-          [&](int m) {
-            const int t = std::max(0, b.np(m) / TILE - s_end);
-            return t * (t + 1) / 2;
-          },
+          b, outer_tiles,
           [&](Prob* const* dp, const Sched& sc, int n) {
             if (tsy) timer_begin(TC_SYRK, s);
-            hipLaunchKernelGGL((k_syrk_outer<T, BF16>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, s0,
-                               s_end);
+            if (fused)
+              hipLaunchKernelGGL((k_syrk_outer<T, BF16, 1>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
+                                 s0, s_end);
+            else
+              hipLaunchKernelGGL((k_syrk_outer<T, BF16, 0>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
+                                 s0, s_end);
             if (tsy) {
               // bytes: every lower tile read and written once, each trailing
               // row block's G panel blocks read once
@@ -603,7 +660,8 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
             }
           });
     }
-    sched_launch(
+    if (!fused)
+      sched_launch(
           b, [&](int m) { return b.np(m) / TILE > k + 1 ? 1 : 0; },
           [&](Prob* const* dp, const Sched& sc, int n) {
             if (tpa) timer_begin(TC_PANEL, s);

@@ -105,7 +105,7 @@ struct Batch {
 // listed member after member (pre[m] = first workgroup of member m), so a
 // launch holds exactly the workgroups that have work, whatever the spread of
 // member sizes.  Passed by value as a kernel argument.
-constexpr int MAXB = 256;  // members per launch; larger batches are split
+constexpr int MAXB = 512;  // members per launch; larger batches are split
 struct Sched {
   int32_t count;
   int32_t pre[MAXB + 1];

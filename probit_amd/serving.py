@@ -19,8 +19,6 @@ import numpy as np
 import torch
 
 from . import _lib
-from .kernels import EQ, Matern12
-from .utilities import likelihood_struct
 
 
 @dataclass
@@ -68,6 +66,7 @@ class ServingEngine:
         self._h = h
         self._slots: dict[int, _Slot] = {}
         self._done_buf = (C.c_int64 * self.capacity)()
+        self._launched = False
 
     # -- sizing ---------------------------------------------------------------
     def request_bytes(self, N: int, D: int, M: int) -> int:
@@ -91,73 +90,124 @@ class ServingEngine:
     def submit(self, tag: int, X, y, kind: str, lengthscale, variance, lik: str, noise_std,
                cutpoints=None, Xtest=None, period=None) -> bool:
         """Queue one posterior; returns False (nothing queued) if it does not fit now.
-        Device tensors are used in place; host arrays are copied to the GPU."""
-        N, D = tuple(X.shape)
+        Device tensors are used in place; host arrays are copied to the GPU.
+        One device block per request holds its workspaces and outputs."""
+        N, D = int(X.shape[0]), int(X.shape[1])
         M = 0 if Xtest is None else int(Xtest.shape[0])
         if not self.can_admit(N, D, M):
             return False
-        dev = self.device
-        t = {}
-        t["X"] = self._dev(X)
-        t["y"] = self._dev(y)
-        t["w"] = torch.empty(N, dtype=torch.float64, device=dev)
-        t["p"] = torch.empty(N, dtype=torch.float64, device=dev)
+        if D > _lib.MAX_DIM:
+            raise ValueError(f"input dimension must be in [1, {_lib.MAX_DIM}] (got {D})")
         nb = self.lib.pgp_workspace_size(N, D, self.prec)
-        t["ws"] = _lib.workspace(nb, dev)
+        pb = self.lib.pgp_predict_workspace_size(N, M, self.prec) if M > 0 else 0
+        vec = (8 * N + 255) // 256 * 256
+        mv = (8 * M + 255) // 256 * 256
+        # [workspace | predict workspace | weight | precision | mean | var]
+        o_pw = (nb + 255) // 256 * 256
+        o_w = o_pw + (pb + 255) // 256 * 256
+        o_p = o_w + vec
+        o_m = o_p + vec
+        o_v = o_m + mv
+        block = torch.empty(o_v + mv, dtype=torch.uint8, device=self.device)
+        base = block.data_ptr()
+        Xd, yd = self._dev(X), self._dev(y)
         item = _lib.BatchItemT()
-        item.problem.N = N
-        item.problem.D = D
-        item.problem.X = t["X"].data_ptr()
-        item.problem.y = t["y"].data_ptr()
-        item.problem.weight = t["w"].data_ptr()
-        item.problem.precision = t["p"].data_ptr()
-        base = EQ() if kind in ("EQ", 0) else Matern12()
-        k = variance * base.stretch(lengthscale)
+        pr = item.problem
+        pr.N, pr.D = N, D
+        pr.X, pr.y = Xd.data_ptr(), yd.data_ptr()
+        pr.weight, pr.precision = base + o_w, base + o_p
+        k = item.kernel
+        k.kind = _lib.KERNEL_EQ if kind in ("EQ", 0) else _lib.KERNEL_MATERN12
+        k.dim = D
+        k.variance = float(variance)
+        ls = np.broadcast_to(np.asarray(lengthscale, dtype=np.float64), (D,))
+        k.lengthscale[:D] = ls.tolist()
         if period is not None:
-            k = k.periodic(period)
-        item.kernel = k.to_struct(D)
-        kind_l = _lib.LIK_GAUSSIAN if lik == "gaussian" else _lib.LIK_PROBIT
-        lp = (noise_std,) if lik == "gaussian" else (noise_std, cutpoints)
-        item.lik = likelihood_struct(kind_l, lp)
-        item.workspace = t["ws"].data_ptr()
+            k.period[:D] = np.broadcast_to(np.asarray(period, dtype=np.float64), (D,)).tolist()
+        lk = item.lik
+        lk.noise_std = float(noise_std)
+        lk.single_precision_bounds = 1
+        if lik == "gaussian":
+            lk.kind = _lib.LIK_GAUSSIAN
+        else:
+            cut = np.asarray(cutpoints, dtype=np.float64).reshape(-1)
+            if cut.shape[0] < 3 or cut.shape[0] > _lib.MAX_CUT:
+                raise ValueError(f"need 3..{_lib.MAX_CUT} cutpoints incl. +-inf")
+            lk.kind = _lib.LIK_PROBIT
+            lk.J = cut.shape[0] - 1
+            lk.cutpoints[:cut.shape[0]] = cut.tolist()
+        item.workspace = base
         item.workspace_bytes = nb
+        keep = [block, Xd, yd]
         if M > 0:
-            t["Xt"] = self._dev(Xtest)
-            t["mean"] = torch.empty(M, dtype=torch.float64, device=dev)
-            t["var"] = torch.empty(M, dtype=torch.float64, device=dev)
-            pb = self.lib.pgp_predict_workspace_size(N, M, self.prec)
-            t["pws"] = _lib.workspace(pb, dev)
-            item.Xtest = t["Xt"].data_ptr()
+            Xt = self._dev(Xtest)
+            keep.append(Xt)
+            item.Xtest = Xt.data_ptr()
             item.M = M
-            item.mean = t["mean"].data_ptr()
-            item.var = t["var"].data_ptr()
-            item.predict_workspace = t["pws"].data_ptr()
+            item.mean, item.var = base + o_m, base + o_v
+            item.predict_workspace = base + o_pw
             item.predict_workspace_bytes = pb
         rc = self.lib.pgp_batcher_submit(self._h, item, int(tag))
         _lib.check(rc, "pgp_batcher_submit")
         s = _Slot()
-        s.item, s.tensors = item, t
+        s.item, s.tensors = item, (keep, N, M, o_w, o_p, o_m, o_v)
         s.nbytes = self.request_bytes(N, D, M)
         s.t_submit = time.perf_counter()
         self._slots[int(tag)] = s
         self.used += s.nbytes
         return True
 
+    @staticmethod
+    def _outputs(t):
+        keep, N, M, o_w, o_p, o_m, o_v = t
+        block = keep[0]
+
+        def f64(off, n):
+            return block[off:off + 8 * n].view(torch.float64)
+
+        mean = f64(o_m, M) if M > 0 else None
+        var = f64(o_v, M) if M > 0 else None
+        return f64(o_w, N), f64(o_p, N), mean, var
+
     def step(self) -> list[Completed]:
         """One Newton iteration for everything in flight; returns finished requests."""
+        if self._launched:
+            raise RuntimeError("step() while an asynchronous launch is pending: call finish()")
         n = C.c_int32(0)
         rc = self.lib.pgp_batcher_step(self._h, self._done_buf, self.capacity, C.byref(n))
         _lib.check(rc, "pgp_batcher_step")
+        return self._collect(n.value)
+
+    def _collect(self, n):
         out = []
-        for i in range(n.value):
+        for i in range(n):
             tag = int(self._done_buf[i])
             s = self._slots.pop(tag)
             r = s.item.result
-            t = s.tensors
-            out.append(Completed(tag, t.get("mean"), t.get("var"), t["w"], t["p"],
-                                 r.iterations, r.status, r.objective))
+            w, p, mean, var = self._outputs(s.tensors)
+            out.append(Completed(tag, mean, var, w, p, r.iterations, r.status, r.objective))
             self.used -= s.nbytes
         return out
+
+    def launch(self) -> bool:
+        """Enqueue one step for everything in flight and return at once
+        (requests submitted meanwhile join at the next launch).  Returns
+        False if there was nothing to launch."""
+        if not self._slots or self._launched:
+            return False
+        _lib.check(self.lib.pgp_batcher_launch(self._h), "pgp_batcher_launch")
+        self._launched = True
+        return True
+
+    def finish(self) -> list[Completed]:
+        """Wait for the launched step; returns the requests it completed."""
+        if not self._launched:
+            return []
+        n = C.c_int32(0)
+        rc = self.lib.pgp_batcher_finish(self._h, self._done_buf, self.capacity, C.byref(n))
+        self._launched = False
+        _lib.check(rc, "pgp_batcher_finish")
+        return self._collect(n.value)
 
     @property
     def in_flight(self) -> int:

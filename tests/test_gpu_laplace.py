@@ -199,3 +199,41 @@ def test_serving_alone_equals_in_mix():
     b = run([0, 1, 2, 3], "bf16")
     assert np.array_equal(a[2].mean.cpu().numpy(), b[2].mean.cpu().numpy())
     assert np.array_equal(a[2].var.cpu().numpy(), b[2].var.cpu().numpy())
+
+
+def test_serving_async_launch_finish_matches_step():
+    """The asynchronous launch / finish pair gives bit-identical results to
+    the synchronous step (items are reported one step later)."""
+    from probit_amd.serving import ServingEngine
+    probs = []
+    for i, (N, D, lik, kind) in enumerate([(150, 2, "probit", "EQ"), (300, 1, "gaussian",
+                                                                      "Matern12"),
+                                           (70, 3, "probit", "Matern12"),
+                                           (260, 2, "gaussian", "EQ")]):
+        X, y, theta, lp, spec = small_problem(60 + i, N, D, lik, kind)
+        Xt = np.random.default_rng(70 + i).normal(size=(90, D))
+        probs.append((X, y, theta, lp, lik, kind, Xt))
+
+    def run(async_mode):
+        eng = ServingEngine(precision="fp32", capacity=3, hbm_budget_bytes=1 << 32)
+        out, queue = {}, list(range(4))
+        while queue or eng.in_flight:
+            launched = eng.launch() if async_mode else False
+            while queue:
+                X, y, theta, lp, lik, kind, Xt = probs[queue[0]]
+                if not eng.submit(queue[0], X, y, kind, theta[0], theta[1], lik, lp[0],
+                                  lp[1] if lik == "probit" else None, Xt):
+                    break
+                queue.pop(0)
+            done = eng.finish() if async_mode and launched else ([] if async_mode else eng.step())
+            for c in done:
+                out[c.tag] = c
+        eng.close()
+        return out
+
+    a, b = run(False), run(True)
+    assert sorted(a) == sorted(b) == [0, 1, 2, 3]
+    for t in range(4):
+        assert a[t].iterations == b[t].iterations
+        for k in ("mean", "var", "weight", "precision"):
+            assert np.array_equal(getattr(a[t], k).cpu().numpy(), getattr(b[t], k).cpu().numpy())
