@@ -158,6 +158,23 @@ __global__ void k_write_ptrs(Prob** dst, int n, PtrPack pk) {
 static_assert(sizeof(Prob) % 8 == 0, "Prob must be 8-byte granular");
 static_assert(sizeof(Prob) <= 2048, "Prob must fit the workspace header");
 
+// Many descriptors at once: `src` (device) holds n staged descriptors,
+// dst[i] is where descriptor i goes.
+__global__ void k_scatter_probs(Prob* const* dst, const Prob* src, int n, int reset) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const uint64_t* sv = reinterpret_cast<const uint64_t*>(src + i);
+  uint64_t* d = reinterpret_cast<uint64_t*>(dst[i]);
+  const int nw = (int)(sizeof(Prob) / 8);
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) d[w] = sv[w];
+  if (reset) {
+    const Prob& v = src[i];
+    if (threadIdx.x < 4) v.info[threadIdx.x] = 0;
+    if (threadIdx.x < NRED) v.red[threadIdx.x] = 0.0;
+    if (threadIdx.x == 0) v.progress[0] = 0;
+  }
+}
+
 static void write_prob(char* ws, const Prob& p, int reset, hipStream_t s) {
   hipLaunchKernelGGL(k_write_prob, dim3(1), dim3(256), 0, s, (Prob*)(ws + OFF_PROB), p, reset);
 }
@@ -864,6 +881,8 @@ struct pgp_batcher {
   double* gather;      // per-member step norms (device)
   double* gather_fin;  // 8 doubles per finalised member (device)
   uint32_t* ticket;
+  Prob* ddesc;            // device staging of descriptors (2 * cap)
+  Prob* hdesc = nullptr;  // pinned host staging of descriptors (2 * cap)
   // asynchronous launch / finish
   double* hstep = nullptr;  // pinned copies of gather / gather_fin
   double* hfin = nullptr;
@@ -880,11 +899,31 @@ struct pgp_batcher {
 namespace pgp {
 
 static int64_t batcher_table_bytes(int cap) {
-  return align256((int64_t)BATCH_ARRAYS * cap * 8) + 2 * align256((int64_t)cap * 8 * 8) + 256;
+  return align256((int64_t)BATCH_ARRAYS * cap * 8) + 2 * align256((int64_t)cap * 8 * 8) + 256 +
+         align256((int64_t)2 * cap * (int64_t)sizeof(Prob));
+}
+
+// Write the descriptors of a batch's members (held in v[i]->p) to their
+// workspaces with one staged copy and one kernel.  `region` (0 or 1) picks
+// the half of the staging buffers, so that two writes of one step do not
+// overlap; `arr` is the batch's device pointer array (already written).
+static void write_probs(pgp_batcher* bt, const std::vector<BItem*>& v, Prob* const* arr,
+                        int region, int reset) {
+  if (v.empty()) return;
+  Prob* h = bt->hdesc + (size_t)region * bt->cap;
+  Prob* d = bt->ddesc + (size_t)region * bt->cap;
+  for (size_t i = 0; i < v.size(); ++i) h[i] = v[i]->p;
+  (void)hipMemcpyAsync(d, h, v.size() * sizeof(Prob), hipMemcpyHostToDevice, bt->s);
+  hipLaunchKernelGGL(k_scatter_probs, dim3((unsigned)v.size()), dim3(256), 0, bt->s, arr, d,
+                     (int)v.size(), reset);
 }
 
 
-static Batch make_batch(pgp_batcher* bt, int which, const std::vector<BItem*>& v) {
+// Members are ordered by padded size, largest first (stable), so that the
+// members still active at any block column form a prefix of the batch.
+static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
+  std::stable_sort(v.begin(), v.end(),
+                   [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
   std::vector<Prob*> ptrs;
   ptrs.reserve(v.size());
   std::vector<int>& hnp = bt->hnp[which];
@@ -921,12 +960,10 @@ static void admit(pgp_batcher* bt) {
   bt->pending.erase(bt->pending.begin(), bt->pending.begin() + take);
   std::vector<BItem*> fresh;
   for (BItem& b : bt->items)
-    if (b.phase == 0) {
-      write_prob(b.ws, b.p, 1, s);
-      fresh.push_back(&b);
-    }
+    if (b.phase == 0) fresh.push_back(&b);
   if (!fresh.empty()) {
     const Batch bn = make_batch(bt, 0, fresh);
+    write_probs(bt, fresh, bn.dprobs, 0, 1);
     launch_gram_self(bn, s);
     launch_vec_op(bn, OP_ZERO_Z, s);
     for (BItem* b : fresh) b->phase = 1;
@@ -935,8 +972,7 @@ static void admit(pgp_batcher* bt) {
 
 // Enqueue the final state (f, W, log p, B factored at W) and the predict of
 // the members in `conv`; 8 result doubles per member land in `gather`.
-static void finalize_and_predict(pgp_batcher* bt, const std::vector<BItem*>& conv,
-                                 double* gather) {
+static void finalize_and_predict(pgp_batcher* bt, std::vector<BItem*>& conv, double* gather) {
   hipStream_t s = bt->s;
   const int prec = bt->opts.precision;
   std::vector<BItem*> conv_fac, pred;
@@ -958,12 +994,12 @@ static void finalize_and_predict(pgp_batcher* bt, const std::vector<BItem*>& con
     b->p.mupart = (double*)(pw + PL.mupart);
     b->p.pmean = it->mean;
     b->p.pvar = it->var;
-    write_prob(b->ws, b->p, 0, s);
     pred.push_back(b);
     maxM = std::max(maxM, (int)it->M);
   }
   if (!pred.empty()) {
     const Batch bp = make_batch(bt, 5, pred);
+    write_probs(bt, pred, bp.dprobs, 1, 0);
     const int maxMp = (int)round_up(maxM, TILE);
     launch_cross_prep(bp, prec, maxM, s);
     launch_trsm(bp, prec, maxMp, 0, s);
@@ -1032,10 +1068,14 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
                             align256((int64_t)capacity * 8 * 8));
   b->ticket = (uint32_t*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
                           2 * align256((int64_t)capacity * 8 * 8));
+  b->ddesc = (Prob*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
+                     2 * align256((int64_t)capacity * 8 * 8) + 256);
   b->hg.resize((size_t)capacity * 8);
   if (hipMemsetAsync(b->ticket, 0, 256, b->s) != hipSuccess ||
       hipHostMalloc((void**)&b->hstep, (size_t)capacity * 8, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&b->hfin, (size_t)capacity * 64, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&b->hdesc, (size_t)2 * capacity * sizeof(Prob),
+                    hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess) {
     pgp_batcher_destroy(b);
     return fail(-100, "batcher set-up (table clear, pinned buffers, event) failed");
@@ -1182,6 +1222,7 @@ int pgp_batcher_destroy(pgp_batcher_t* bt) {
   if (bt->ev) (void)hipEventDestroy(bt->ev);
   if (bt->hstep) (void)hipHostFree(bt->hstep);
   if (bt->hfin) (void)hipHostFree(bt->hfin);
+  if (bt->hdesc) (void)hipHostFree(bt->hdesc);
   delete bt;
   return 0;
 }

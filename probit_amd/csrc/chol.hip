@@ -231,31 +231,33 @@ __global__ __launch_bounds__(NTHREADS) void k_factor_first(Prob* const* probs) {
 }
 
 // ---- P_k: panel rows below block k --------------------------------------------
+// L_ik = A_ik DINV_k^T in the storage type: FP64 MFMA for an FP64 factor,
+// FP32 MFMA (the FP64 inverse rounded to FP32) for an FP32 one, whose
+// trailing updates carry FP32 rounding anyway.
 template <typename T>
 __device__ void panel_rows(T* A, int64_t ld, const double* dinv_k, int64_t c0, int64_t r0) {
-  __shared__ double As[TILE * (TILE + 1)];
-  __shared__ double Bs[TILE * (TILE + 1)];
+  __shared__ T As[TILE * (TILE + 1)];
+  __shared__ T Bs[TILE * (TILE + 1)];
   // As[p][m] = A(r0 + m, c0 + p); Bs[p][n] = DINV_k(n, p)
-  load_block_kmajor<T, double>(As, TILE + 1, A, ld, r0, c0);
+  load_block_kmajor<T, T>(As, TILE + 1, A, ld, r0, c0);
   {
     const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
     for (int q = 0; q < 16; ++q) {
       const int p = pb + 4 * q;
-      Bs[p * (TILE + 1) + n] = dinv_k[n + p * TILE];
+      Bs[p * (TILE + 1) + n] = (T)dinv_k[n + p * TILE];
     }
   }
   __syncthreads();
-  doublex4 acc[2][2];
-  zero_acc<double>(acc);
-  mma64_kmajor<double>(As, TILE + 1, Bs, TILE + 1, acc);
+  typename Acc<T>::type acc[2][2];
+  zero_acc<T>(acc);
+  mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        A[(size_t)(r0 + acc_row<double>(bm, r)) + (size_t)(c0 + acc_col(bn)) * ld] =
-            (T)acc[bm][bn][r];
+        A[(size_t)(r0 + acc_row<T>(bm, r)) + (size_t)(c0 + acc_col(bn)) * ld] = (T)acc[bm][bn][r];
 }
 
 template <typename T>
@@ -455,6 +457,140 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
                                  (int64_t)(s_end + tj) * TILE, FUSE && ti == 0 && tj == 0, o);
 }
 
+// ---- one launch per block column ---------------------------------------------
+// C = A(ri.., rj..) - sum_{c in [kc0, kc1)} A(ri.., 64c..) A(rj.., 64c..)^T,
+// computed as the trailing updates are (T / BF16 operands on MFMA, result
+// rounded to T), then held in LDS as FP64: out[r][c] (kmajor = false) or
+// out[c][r] (kmajor = true).  `arena` holds the operands first.
+template <typename T, int BF16, typename OT>
+__device__ void update_tile_to_lds(const T* A, int64_t ld, int kc0, int kc1, int64_t ri,
+                                   int64_t rj, char* arena, OT (*out)[TILE + 1], bool kmajor) {
+  constexpr int PITCH = BF16 ? 72 : TILE + 1;
+  typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
+  typedef typename Acc<T>::type acc_t;
+  LT* sa = (LT*)arena;
+  LT* sb = sa + TILE * PITCH;
+  const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
+  acc_t acc[2][2];
+  zero_acc<T>(acc);
+  if (kc1 > kc0) {
+    T ra[16], rb[16];
+    auto fetch = [&](int c) {
+      const int64_t c0 = (int64_t)c * TILE;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = kb + 4 * q;
+        ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
+        rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
+      }
+    };
+    fetch(kc0);
+    for (int c = kc0; c < kc1; ++c) {
+      if (c > kc0) __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = kb + 4 * q;
+        if constexpr (BF16) {
+          sa[m * PITCH + k] = f2bf((float)ra[q]);
+          sb[m * PITCH + k] = f2bf((float)rb[q]);
+        } else {
+          sa[k * PITCH + m] = ra[q];
+          sb[k * PITCH + m] = rb[q];
+        }
+      }
+      __syncthreads();
+      if (c + 1 < kc1) fetch(c + 1);
+      if constexpr (BF16)
+        mma64_bf16(sa, PITCH, sb, PITCH, acc);
+      else
+        mma64_kmajor<T>(sa, PITCH, sb, PITCH, acc);
+    }
+    __syncthreads();  // operands are dead: `out` may overlay them
+  }
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) {
+      const int col = acc_col(bn);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = acc_row<T>(bm, r);
+        const T v = A[(size_t)(ri + row) + (size_t)(rj + col) * ld] - (T)acc[bm][bn][r];
+        if (kmajor)
+          out[col][row] = (OT)v;
+        else
+          out[row][col] = (OT)v;
+      }
+    }
+  __syncthreads();
+}
+
+// Column j of every member in one launch.  Tickets [0, nd) are the diagonal
+// workgroups of members 0..nd-1 (members are sorted by size, so these are
+// the members with more than j block columns); the rest are row workgroups
+// listed by `rows`.  Within the super-step [s0, ...), column j has not yet
+// received the updates of columns s0..j-1; each workgroup applies them to
+// its own tile (left-looking, rank 64 (j - s0)).  The diagonal workgroup
+// then factors and inverts block j and publishes; a row workgroup waits for
+// its member's publication and forms L_ij = C_ij DINV_j^T (FP64 MFMA).
+template <typename T, int BF16>
+__global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched rows, int nd,
+                                                     uint32_t* ticket, uint32_t epoch, int j,
+                                                     int s0) {
+  __shared__ __attribute__((aligned(16))) double arena[2][TILE][TILE + 1];
+  __shared__ int s_val;
+  const int t = take_ticket(ticket, &s_val);
+  if ((unsigned)t >= gridDim.x) return;  // counter not zero at launch
+  const int64_t c0 = (int64_t)j * TILE;
+  if (t < nd) {
+    const Prob& P = *probs[t];
+    update_tile_to_lds<T, BF16, double>((const T*)P.L, P.Np, s0, j, c0, c0, (char*)arena,
+                                        arena[0], false);
+    FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
+                P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
+                P.info};
+    factor_block_blocked<T>(arena[0], arena[1], c0, o);
+    // every wave's stores complete and written back, then publish
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(P.progress, ((uint64_t)epoch << 32) | 1u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  int local;
+  const Prob& P = *probs[sched_member(rows, t - nd, local)];
+  const int64_t ri = c0 + (int64_t)TILE * (local + 1);
+  // the panel product runs in the storage type (as panel_rows)
+  T (*ck)[TILE + 1] = (T (*)[TILE + 1]) & arena[0][0][0];
+  T (*bk)[TILE + 1] = ck + TILE;
+  update_tile_to_lds<T, BF16, T>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena, ck, true);
+  if (threadIdx.x == 0) wait_progress(P.progress, epoch, 1, P.info);
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // B[p][n] = DINV_j(n, p)
+  const double* Dj = P.dinv + (size_t)j * TILE * TILE;
+  {
+    const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
+    for (int q = 0; q < 16; ++q) {
+      const int pp = pb + 4 * q;
+      bk[pp][n] = (T)Dj[n + pp * TILE];
+    }
+  }
+  __syncthreads();
+  typename Acc<T>::type acc[2][2];
+  zero_acc<T>(acc);
+  mma64_kmajor<T>(&ck[0][0], TILE + 1, &bk[0][0], TILE + 1, acc);
+  T* L = (T*)P.L;
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        L[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(c0 + acc_col(bn)) * P.Np] = acc[bm][bn][r];
+}
+
 // Diagonal block k (k >= 1) after the trailing update of step k - 1 (separate
 // mode): one workgroup per member.
 template <typename T>
@@ -560,12 +696,62 @@ static int superstep() {
   return v;
 }
 
+// One launch per block column (k_column) for the super-steps whose first
+// column has at most this many workgroups (PGP_COLUMN_MAX; 0 disables):
+// there the launch count and the per-column chain dominate; larger
+// super-steps use the separate panel / update / factor kernels, whose lean
+// register footprint keeps occupancy high.
+static int column_max() {
+  static const int v = [] {
+    const char* e = getenv("PGP_COLUMN_MAX");
+    return e ? atoi(e) : 512;
+  }();
+  return v;
+}
+
+static bool sorted_desc(const Batch& b) {
+  for (int m = 1; m < b.count; ++m)
+    if (b.np(m) > b.np(m - 1)) return false;
+  return true;
+}
+
+template <typename T, int BF16>
+static void column_launch(const Batch& b, int j, int s0, bool tpa, hipStream_t s) {
+  constexpr double F3 = TILE * TILE * TILE / 3.0;
+  constexpr double G64 = 2.0 * TILE * TILE * TILE;
+  for (int off = 0; off < b.count; off += MAXB) {
+    const int n = std::min(MAXB, b.count - off);
+    Sched sc;
+    sc.count = n;
+    int tot = 0, nd = 0;
+    double fl = 0.0;
+    for (int m = 0; m < n; ++m) {
+      const int nbm = b.np(off + m) / TILE;
+      sc.pre[m] = tot;
+      if (nbm > j) {
+        ++nd;
+        tot += nbm - j - 1;
+        fl += F3 + (double)(nbm - j) * G64 * (j - s0) + (double)(nbm - j - 1) * G64;
+      }
+    }
+    sc.pre[n] = tot;
+    if (nd == 0) continue;
+    const uint32_t ep = next_epoch();
+    if (tpa) timer_begin(TC_PANEL, s);
+    hipLaunchKernelGGL((k_column<T, BF16>), dim3(nd + tot), dim3(NTHREADS), 0, s, b.dprobs + off,
+                       sc, nd, b.ticket, ep, j, s0);
+    if (tpa) timer_end(TC_PANEL, s, fl, 0.0);
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Per block column k (every member at once):
-//   P_k   panel below the diagonal block;
-//   I_k   inner update of the remaining columns of k's super-step;
-//   O     (last column of a super-step) outer update of everything right of it;
-//   F_k+1 factor (and invert) the next diagonal block.
+// Per block column k (every member at once), in super-steps of G columns:
+//   separate kernels: P_k panel; I_k inner update of the remaining columns of
+//     k's super-step (or, at its last column, O the outer update of
+//     everything right of it); F_k+1 factor the next diagonal block (inside
+//     I_k / O when that launch is small);
+//   column kernel: one k_column launch per column (left-looking update,
+//     factor, panel), then O at the super-step's end.
 template <typename T, int BF16>
 static void potrf_steps(const Batch& b, hipStream_t s) {
   const int nb = b.maxNp / TILE;
@@ -580,12 +766,65 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
     for (int m = off; m < off + sc.count; ++m) f += per_member(b.np(m) / TILE);
     return f;
   };
-  if (tpa) timer_begin(TC_PANEL, s);
-  hipLaunchKernelGGL(k_factor_first<T>, dim3(1, b.count), dim3(NTHREADS), 0, s, b.dprobs);
-  if (tpa) timer_end(TC_PANEL, s, b.count * F3, 0.0);
-  for (int k = 0; k + 1 < nb; ++k) {
+  const bool colok = column_max() > 0 && b.ticket && sorted_desc(b);
+  // workgroups of the column launch at the first column of a super-step;
+  // non-increasing in s0, so once a super-step uses the column kernel all
+  // later ones do
+  auto col_style = [&](int s0) {
+    if (!colok) return false;
+    long long w = 0;
+    for (int m = 0; m < b.count; ++m) w += std::max(0, b.np(m) / TILE - s0);
+    return w <= column_max();
+  };
+  auto outer_launch = [&](int s0, int s_end, bool fuse) {
+    auto outer_tiles = [&](int m) {
+      const int t = std::max(0, b.np(m) / TILE - s_end);
+      return t * (t + 1) / 2;
+    };
+    sched_launch(b, outer_tiles, [&](Prob* const* dp, const Sched& sc, int n) {
+      if (tsy) timer_begin(TC_SYRK, s);
+      if (fuse)
+        hipLaunchKernelGGL((k_syrk_outer<T, BF16, 1>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, s0,
+                           s_end);
+      else
+        hipLaunchKernelGGL((k_syrk_outer<T, BF16, 0>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, s0,
+                           s_end);
+      if (tsy) {
+        // outer update: an n x n lower triangle (n = 64 T) with rank K = 64 G
+        // counts n (n + 1) K flops; bytes: every lower tile read and written
+        // once, each trailing row block's G panel blocks read once
+        const double es = sizeof(T);
+        timer_end(TC_SYRK, s, flops(sc, dp,
+                                    [&](int nbm) {
+                                      const double nn = (double)std::max(0, nbm - s_end) * TILE;
+                                      return nn * (nn + 1) * (double)(TILE * G);
+                                    }),
+                  flops(sc, dp, [&](int nbm) {
+                    const double t = std::max(0, nbm - s_end);
+                    return (t * (t + 1) * TILE * TILE + t * TILE * TILE * G) * es;
+                  }));
+      }
+    });
+  };
+  auto total_units = [&](auto units) {
+    int t = 0;
+    for (int m = 0; m < b.count; ++m) t += units(m);
+    return t;
+  };
+  if (!col_style(0)) {
+    if (tpa) timer_begin(TC_PANEL, s);
+    hipLaunchKernelGGL(k_factor_first<T>, dim3(1, b.count), dim3(NTHREADS), 0, s, b.dprobs);
+    if (tpa) timer_end(TC_PANEL, s, b.count * F3, 0.0);
+  }
+  for (int k = 0; k < nb; ++k) {
     const int s0 = k / G * G, s_end = s0 + G;
     const bool outer = (k + 1 == s_end);
+    if (col_style(s0)) {
+      column_launch<T, BF16>(b, k, s0, tpa, s);
+      if (outer && s_end < nb) outer_launch(s0, s_end, false);
+      continue;
+    }
+    if (k + 1 >= nb) break;  // the last diagonal block is already factored
     sched_launch(
         b, [&](int m) { return std::max(0, b.np(m) / TILE - k - 1); },
         [&](Prob* const* dp, const Sched& sc, int n) {
@@ -595,11 +834,11 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
             timer_end(TC_PANEL, s,
                       flops(sc, dp, [&](int nbm) { return std::max(0, nbm - k - 1) * G64; }), 0.0);
         });
-    // Small update launches (late columns, few members) carry the next
-    // diagonal factorisation in the workgroup of tile (k+1, k+1): one launch
-    // fewer on the critical chain.  Large ones keep the lean kernel (the
-    // factorisation's registers and LDS would lower its occupancy) and a
-    // separate factor launch.
+    // Small update launches carry the next diagonal factorisation in the
+    // workgroup of tile (k+1, k+1): one launch fewer on the critical chain.
+    // Large ones keep the lean kernel and a separate factor launch.  If the
+    // next super-step uses the column kernel, that kernel factors block k+1.
+    const bool next_col = outer && col_style(s_end);
     bool fused = false;
     if (!outer) {
       auto inner_tiles = [&](int nbm) {
@@ -607,9 +846,7 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
         for (int j = k + 1; j < std::min(s_end, nbm); ++j) t += nbm - j;
         return t;
       };
-      int total = 0;
-      for (int m = 0; m < b.count; ++m) total += inner_tiles(b.np(m) / TILE);
-      fused = total <= fuse_max();
+      fused = total_units([&](int m) { return inner_tiles(b.np(m) / TILE); }) <= fuse_max();
       sched_launch(
           b, [&](int m) { return inner_tiles(b.np(m) / TILE); },
           [&](Prob* const* dp, const Sched& sc, int n) {
@@ -625,42 +862,13 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
                         0.0);
           });
     } else {
-      // outer update: an n x n lower triangle (n = 64 T) with rank K = 64 G
-      // counts n (n + 1) K flops
-      auto outer_tiles = [&](int m) {
-        const int t = std::max(0, b.np(m) / TILE - s_end);
-        return t * (t + 1) / 2;
-      };
-      int total = 0;
-      for (int m = 0; m < b.count; ++m) total += outer_tiles(m);
-      fused = total <= fuse_max();
-      sched_launch(
-          b, outer_tiles,
-          [&](Prob* const* dp, const Sched& sc, int n) {
-            if (tsy) timer_begin(TC_SYRK, s);
-            if (fused)
-              hipLaunchKernelGGL((k_syrk_outer<T, BF16, 1>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
-                                 s0, s_end);
-            else
-              hipLaunchKernelGGL((k_syrk_outer<T, BF16, 0>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
-                                 s0, s_end);
-            if (tsy) {
-              // bytes: every lower tile read and written once, each trailing
-              // row block's G panel blocks read once
-              const double es = sizeof(T);
-              timer_end(TC_SYRK, s, flops(sc, dp,
-                                          [&](int nbm) {
-                                            const double nn = (double)std::max(0, nbm - s_end) * TILE;
-                                            return nn * (nn + 1) * (double)(TILE * G);
-                                          }),
-                        flops(sc, dp, [&](int nbm) {
-                          const double t = std::max(0, nbm - s_end);
-                          return (t * (t + 1) * TILE * TILE + t * TILE * TILE * G) * es;
-                        }));
-            }
-          });
+      fused = !next_col && total_units([&](int m) {
+                             const int t = std::max(0, b.np(m) / TILE - s_end);
+                             return t * (t + 1) / 2;
+                           }) <= fuse_max();
+      outer_launch(s0, s_end, fused);
     }
-    if (!fused)
+    if (!fused && !next_col)
       sched_launch(
           b, [&](int m) { return b.np(m) / TILE > k + 1 ? 1 : 0; },
           [&](Prob* const* dp, const Sched& sc, int n) {

@@ -144,6 +144,54 @@ inline void sched_launch(const Batch& b, Units units, Go go) {
   }
 }
 
+// Cross-workgroup publication inside one launch (wavefront solves, column
+// steps of the factorisation): a workgroup only waits for workgroups that
+// took a lower ticket (already running), and every wait is bounded.
+constexpr int SPIN_LIMIT = 1 << 22;
+uint32_t next_epoch();  // fresh epoch per launch (never 0)
+
+// Relaxed agent-scope (L2-coherent) accesses for values handed between
+// workgroups of one launch.
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Ticket of this workgroup in start order (thread 0 takes it; the last
+// ticket resets the counter for the next launch on the stream).
+__device__ __forceinline__ int take_ticket(uint32_t* ticket, int* lds_slot) {
+  if (threadIdx.x == 0) {
+    const uint32_t t = atomicAdd(ticket, 1u);
+    if (t == gridDim.x - 1) atomicExch(ticket, 0u);
+    *lds_slot = (int)t;
+  }
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*lds_slot);
+}
+
+// Wait (thread 0, bounded) until a member's progress word shows at least
+// `need` published steps of launch `epoch`; flags PGP_INFO_STALLED if the
+// bound runs out.  Returns the published count seen.
+__device__ __forceinline__ int wait_progress(uint64_t* progress, uint32_t epoch, int need,
+                                             int32_t* info) {
+  int r = 0;
+  for (int spins = 0;; ++spins) {
+    const uint64_t pv = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r = ((uint32_t)(pv >> 32) == epoch) ? (int)(uint32_t)pv : 0;
+    if (r >= need) break;
+    if (spins >= SPIN_LIMIT) {
+      atomicOr(info, PGP_INFO_STALLED);
+      r = need;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return r;
+}
+
 // Opt-in launch timer: HIP events around the launches of one kernel class
 // on the launching stream, with the algorithmic flops each launch performs.
 //   TC_SYRK  outer (rank 64 G) trailing update of the Cholesky

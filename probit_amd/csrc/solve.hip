@@ -116,14 +116,6 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv(Prob* const* probs, Sched sc,
 // workgroups that are already running: the wait always ends.  A wait is
 // also bounded (SPIN_LIMIT polls); if the bound is ever reached the member
 // is flagged PGP_INFO_STALLED and the sweep runs on, so every workgroup exits.
-constexpr int SPIN_LIMIT = 1 << 22;
-
-__device__ __forceinline__ double ld_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_coherent(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sched sc,
@@ -276,7 +268,7 @@ static bool wavefront_trsv() {
   return v;
 }
 
-static uint32_t next_epoch() {
+uint32_t next_epoch() {
   static std::atomic<uint32_t> g{0};
   uint32_t e = ++g;
   if (e == 0) e = ++g;
