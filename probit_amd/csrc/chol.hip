@@ -236,21 +236,22 @@ __global__ __launch_bounds__(NTHREADS) void k_factor_first(Prob* const* probs) {
 // trailing updates carry FP32 rounding anyway.
 template <typename T>
 __device__ void panel_rows(T* A, int64_t ld, const double* dinv_k, int64_t c0, int64_t r0) {
-  __shared__ T As[TILE * (TILE + 1)];
-  __shared__ T Bs[TILE * (TILE + 1)];
+  constexpr int PP = KP<T>::value;
+  __shared__ T As[TILE * PP];
+  __shared__ T Bs[TILE * PP];
   // As[p][m] = A(r0 + m, c0 + p); Bs[p][n] = DINV_k(n, p)
-  load_block_kmajor<T, T>(As, TILE + 1, A, ld, r0, c0);
+  load_block_kmajor<T, T>(As, PP, A, ld, r0, c0);
   {
     const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
     for (int q = 0; q < 16; ++q) {
       const int p = pb + 4 * q;
-      Bs[p * (TILE + 1) + n] = (T)dinv_k[n + p * TILE];
+      Bs[p * PP + n] = (T)dinv_k[n + p * TILE];
     }
   }
   __syncthreads();
   typename Acc<T>::type acc[2][2];
   zero_acc<T>(acc);
-  mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
+  mma64_kmajor<T>(As, PP, Bs, PP, acc);
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -276,12 +277,22 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, 
                           const FactorOut& o) {
   // one LDS arena: MMA operands, then (fused mode, tile (k+1, k+1) only) the
   // two FP64 blocks of the diagonal factorisation
-  constexpr int OPER = BF16 ? 2 * TILE * 72 * 2 : 2 * TILE * (TILE + 1) * (int)sizeof(T);
+  constexpr int PP = KP<T>::value;
+  constexpr int OPER = BF16 ? 2 * TILE * 72 * 2 : 2 * TILE * PP * (int)sizeof(T);
   constexpr int BLK = FUSE ? 2 * TILE * (TILE + 1) * 8 : 0;
   __shared__ __attribute__((aligned(16))) char arena[OPER > BLK ? OPER : BLK];
   typedef typename Acc<T>::type acc_t;
   acc_t acc[2][2];
   zero_acc<T>(acc);
+  // the C tile is loaded first, so its latency overlaps the product
+  T cpre[2][2][4];
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        cpre[bm][bn][r] = A[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(rj + acc_col(bn)) * ld];
   if constexpr (BF16) {
     unsigned short* si = (unsigned short*)arena;
     unsigned short* sj = si + TILE * 72;
@@ -291,11 +302,11 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, 
     mma64_bf16(si, 72, sj, 72, acc);
   } else {
     T* si = (T*)arena;
-    T* sj = si + TILE * (TILE + 1);
-    load_block_kmajor<T, T>(si, TILE + 1, A, ld, ri, c0);
-    load_block_kmajor<T, T>(sj, TILE + 1, A, ld, rj, c0);
+    T* sj = si + TILE * PP;
+    load_block_kmajor<T, T>(si, PP, A, ld, ri, c0);
+    load_block_kmajor<T, T>(sj, PP, A, ld, rj, c0);
     __syncthreads();
-    mma64_kmajor<T>(si, TILE + 1, sj, TILE + 1, acc);
+    mma64_kmajor<T>(si, PP, sj, PP, acc);
   }
   if constexpr (!FUSE) factor = false;
   double (*blk)[TILE + 1] = (double (*)[TILE + 1])arena;
@@ -308,12 +319,11 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = acc_row<T>(bm, r);
-        T* p = &A[(size_t)(ri + row) + (size_t)(rj + col) * ld];
-        const T v = *p - acc[bm][bn][r];
+        const T v = cpre[bm][bn][r] - acc[bm][bn][r];
         if (factor)
           blk[row][col] = (double)v;
         else
-          *p = v;
+          A[(size_t)(ri + row) + (size_t)(rj + col) * ld] = v;
       }
     }
   if constexpr (FUSE) {
@@ -343,7 +353,7 @@ __device__ __forceinline__ int xcd_contiguous(int g, int total) {
 template <typename T, int BF16, int FUSE>
 __device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj,
                                 bool factor, const FactorOut& o) {
-  constexpr int PITCH = BF16 ? 72 : TILE + 1;
+  constexpr int PITCH = BF16 ? 72 : KP<T>::value;
   typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
   constexpr int OPER = 2 * TILE * PITCH * (int)sizeof(LT);
   constexpr int BLK = FUSE ? 2 * TILE * (TILE + 1) * 8 : 0;
@@ -365,25 +375,40 @@ __device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, 
     }
   };
   fetch(kc0);
-  for (int c = kc0; c < kc1; ++c) {
-    if (c > kc0) __syncthreads();  // the previous chunk's products are done with LDS
+  // the C tile is loaded up front, so its latency overlaps the product
+  T cpre[2][2][4];
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        cpre[bm][bn][r] = A[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(rj + acc_col(bn)) * ld];
+  auto stash = [&](LT* da, LT* db) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = kb + 4 * q;
       if constexpr (BF16) {
-        sa[m * PITCH + k] = f2bf((float)ra[q]);
-        sb[m * PITCH + k] = f2bf((float)rb[q]);
+        da[m * PITCH + k] = f2bf((float)ra[q]);
+        db[m * PITCH + k] = f2bf((float)rb[q]);
       } else {
-        sa[k * PITCH + m] = ra[q];
-        sb[k * PITCH + m] = rb[q];
+        da[k * PITCH + m] = ra[q];
+        db[k * PITCH + m] = rb[q];
       }
     }
+  };
+  auto mma = [&](const LT* da, const LT* db) {
+    if constexpr (BF16)
+      mma64_bf16(da, PITCH, db, PITCH, acc);
+    else
+      mma64_kmajor<T>(da, PITCH, db, PITCH, acc);
+  };
+  for (int c = kc0; c < kc1; ++c) {
+    if (c > kc0) __syncthreads();  // the previous chunk's products are done with LDS
+    stash(sa, sb);
     __syncthreads();
     if (c + 1 < kc1) fetch(c + 1);
-    if constexpr (BF16)
-      mma64_bf16(sa, PITCH, sb, PITCH, acc);
-    else
-      mma64_kmajor<T>(sa, PITCH, sb, PITCH, acc);
+    mma(sa, sb);
   }
   if constexpr (!FUSE) factor = false;
   double (*blk)[TILE + 1] = (double (*)[TILE + 1])arena;
@@ -396,12 +421,11 @@ __device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = acc_row<T>(bm, r);
-        T* p = &A[(size_t)(ri + row) + (size_t)(rj + col) * ld];
-        const T v = *p - (T)acc[bm][bn][r];
+        const T v = cpre[bm][bn][r] - (T)acc[bm][bn][r];
         if (factor)
           blk[row][col] = (double)v;
         else
-          *p = v;
+          A[(size_t)(ri + row) + (size_t)(rj + col) * ld] = v;
       }
     }
   if constexpr (FUSE) {
@@ -454,7 +478,7 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
                     P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr,
                     P.info};
   syrk_tile_multi<T, BF16, FUSE>((T*)P.L, P.Np, s0, s_end, (int64_t)(s_end + ti) * TILE,
-                                 (int64_t)(s_end + tj) * TILE, FUSE && ti == 0 && tj == 0, o);
+                                     (int64_t)(s_end + tj) * TILE, FUSE && ti == 0 && tj == 0, o);
 }
 
 // ---- one launch per block column ---------------------------------------------
@@ -464,8 +488,8 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
 // out[c][r] (kmajor = true).  `arena` holds the operands first.
 template <typename T, int BF16, typename OT>
 __device__ void update_tile_to_lds(const T* A, int64_t ld, int kc0, int kc1, int64_t ri,
-                                   int64_t rj, char* arena, OT (*out)[TILE + 1], bool kmajor) {
-  constexpr int PITCH = BF16 ? 72 : TILE + 1;
+                                   int64_t rj, char* arena, OT* out, int opitch, bool kmajor) {
+  constexpr int PITCH = BF16 ? 72 : KP<T>::value;
   typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
   typedef typename Acc<T>::type acc_t;
   LT* sa = (LT*)arena;
@@ -517,9 +541,9 @@ __device__ void update_tile_to_lds(const T* A, int64_t ld, int kc0, int kc1, int
         const int row = acc_row<T>(bm, r);
         const T v = A[(size_t)(ri + row) + (size_t)(rj + col) * ld] - (T)acc[bm][bn][r];
         if (kmajor)
-          out[col][row] = (OT)v;
+          out[col * opitch + row] = (OT)v;
         else
-          out[row][col] = (OT)v;
+          out[row * opitch + col] = (OT)v;
       }
     }
   __syncthreads();
@@ -545,7 +569,7 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
   if (t < nd) {
     const Prob& P = *probs[t];
     update_tile_to_lds<T, BF16, double>((const T*)P.L, P.Np, s0, j, c0, c0, (char*)arena,
-                                        arena[0], false);
+                                        &arena[0][0][0], TILE + 1, false);
     FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
                 P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
                 P.info};
@@ -562,9 +586,10 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
   const Prob& P = *probs[sched_member(rows, t - nd, local)];
   const int64_t ri = c0 + (int64_t)TILE * (local + 1);
   // the panel product runs in the storage type (as panel_rows)
-  T (*ck)[TILE + 1] = (T (*)[TILE + 1]) & arena[0][0][0];
-  T (*bk)[TILE + 1] = ck + TILE;
-  update_tile_to_lds<T, BF16, T>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena, ck, true);
+  constexpr int PP = KP<T>::value;
+  T* ck = (T*)&arena[0][0][0];
+  T* bk = ck + TILE * PP;
+  update_tile_to_lds<T, BF16, T>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena, ck, PP, true);
   if (threadIdx.x == 0) wait_progress(P.progress, epoch, 1, P.info);
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -574,13 +599,13 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
     const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
     for (int q = 0; q < 16; ++q) {
       const int pp = pb + 4 * q;
-      bk[pp][n] = (T)Dj[n + pp * TILE];
+      bk[pp * PP + n] = (T)Dj[n + pp * TILE];
     }
   }
   __syncthreads();
   typename Acc<T>::type acc[2][2];
   zero_acc<T>(acc);
-  mma64_kmajor<T>(&ck[0][0], TILE + 1, &bk[0][0], TILE + 1, acc);
+  mma64_kmajor<T>(ck, PP, bk, PP, acc);
   T* L = (T*)P.L;
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)

@@ -23,6 +23,15 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
+// Pitch (elements) of a k-major 64-column LDS operand.  For 4-byte types the
+// four k-groups of a 16x16x4 MFMA operand read (lanes 16q.., rows r..r+15,
+// k = kk + q) then fall on disjoint LDS banks (pitch = 16 mod 32 banks);
+// 8-byte types already do with 65.
+template <typename T>
+struct KP {
+  static constexpr int value = sizeof(T) == 4 ? 80 : 65;
+};
+
 template <typename T>
 struct Acc;
 template <>

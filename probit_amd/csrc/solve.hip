@@ -121,7 +121,7 @@ template <typename T>
 __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sched sc,
                                                         uint32_t* ticket, uint32_t epoch,
                                                         int trans, int in, int out) {
-  __shared__ double dk[TILE * TILE];  // DINV_i, column-major
+  __shared__ double dk[TILE * (TILE + 1)];  // DINV_i, column-major, column pitch 65
   __shared__ double part[4][TILE];
   __shared__ double v[TILE];
   __shared__ int s_val;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sche
   const double* b = vslot(P, in);
   double* x = vslot(P, out);
   const double* D = P.dinv + (size_t)i * TILE * TILE;
-  for (int e = threadIdx.x; e < TILE * TILE; e += NTHREADS) dk[e] = D[e];
+  for (int e = threadIdx.x; e < TILE * TILE; e += NTHREADS) dk[(e >> 6) * (TILE + 1) + (e & 63)] = D[e];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t ri = (int64_t)i * TILE;
   double pf = 0.0;      // forward: row l, columns 16w.. of each tile
@@ -248,7 +248,8 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sche
   // x_i = DINV_i v (forward) or DINV_i^T v (transposed)
   double a = 0.0;
 #pragma unroll
-  for (int q = 16 * w; q < 16 * w + 16; ++q) a += (trans ? dk[q + l * TILE] : dk[l + q * TILE]) * v[q];
+  for (int q = 16 * w; q < 16 * w + 16; ++q)
+    a += (trans ? dk[q + l * (TILE + 1)] : dk[l + q * (TILE + 1)]) * v[q];
   part[w][l] = a;
   __syncthreads();
   if (threadIdx.x < TILE) {  // wave 0: store x_i, one release fence, then publish
@@ -364,8 +365,10 @@ int launch_trsv_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const d
 template <typename T>
 __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* S, int64_t lds,
                           int w, int ct, int lower_rhs) {
-  __shared__ T As[TILE * (TILE + 1)];
-  __shared__ T Bs[TILE * (TILE + 1)];
+  constexpr int PP = KP<T>::value;  // A operand: conflict-free MFMA reads
+  constexpr int PB = TILE + 1;       // B operand: written transposed, conflict-free stores
+  __shared__ T As[TILE * PP];
+  __shared__ T Bs[TILE * PB];
   typedef typename Acc<T>::type acc_t;
   if (lower_rhs && (ct > k || (ct == k && w > 0))) return;
   const int64_t r0 = (int64_t)(k + w) * TILE;
@@ -375,17 +378,17 @@ __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* 
   if (k > 0) {
     const int64_t cp = (int64_t)(k - 1) * TILE;
     // A = L(r0.., cp..) (k-major), B[kk][n] = S(cp + kk, n0 + n)
-    load_block_kmajor<T, T>(As, TILE + 1, L, ld, r0, cp);
+    load_block_kmajor<T, T>(As, PP, L, ld, r0, cp);
     {
       const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
 #pragma unroll 4
       for (int q = 0; q < 16; ++q) {
         const int c = cb + 4 * q;
-        Bs[r * (TILE + 1) + c] = S[(size_t)(cp + r) + (size_t)(n0 + c) * lds];
+        Bs[r * PB + c] = S[(size_t)(cp + r) + (size_t)(n0 + c) * lds];
       }
     }
     __syncthreads();
-    mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
+    mma64_kmajor<T>(As, PP, Bs, PB, acc);
     __syncthreads();
   }
   if (w > 0) {
@@ -408,13 +411,13 @@ __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = acc_row<T>(bm, r), col = acc_col(bn);
-        Bs[row * (TILE + 1) + col] =
+        Bs[row * PB + col] =
             S[(size_t)(r0 + row) + (size_t)(n0 + col) * lds] - acc[bm][bn][r];
       }
-  load_block_kmajor<double, T>(As, TILE + 1, dinv + (size_t)k * TILE * TILE, TILE, 0, 0);
+  load_block_kmajor<double, T>(As, PP, dinv + (size_t)k * TILE * TILE, TILE, 0, 0);
   __syncthreads();
   zero_acc<T>(acc);
-  mma64_kmajor<T>(As, TILE + 1, Bs, TILE + 1, acc);
+  mma64_kmajor<T>(As, PP, Bs, PB, acc);
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
