@@ -436,28 +436,25 @@ __device__ void syrk_tile_multi(T* A, int64_t ld, int kc0, int kc1, int64_t ri, 
   }
 }
 
-// Inner update of a super-step: after panel k, the block columns j of the
-// same super-step (k < j < s_end) receive A_ij -= L_ik L_jk^T for i >= j.
+// Left-looking update of block column j inside its super-step [s0, ...):
+// tiles (i, j), i >= j, receive A_ij -= sum_{c in [s0, j)} L_ic L_jc^T in one
+// product (the same arithmetic as the column kernel's update), just before
+// block j is factored and its panel formed.  FUSE: the workgroup of tile
+// (j, j) factors that block at once.
 template <typename T, int BF16, int FUSE>
-__global__ __launch_bounds__(NTHREADS) void k_syrk_inner(Prob* const* probs, Sched sc, int k,
-                                                         int s_end) {
+__global__ __launch_bounds__(NTHREADS) void k_col_update(Prob* const* probs, Sched sc, int j,
+                                                         int s0) {
   int bx;
   const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int nb = P.Np / TILE;
-  const int e = s_end < nb ? s_end : nb;
-  int j = k + 1;
-  while (j < e && bx >= nb - j) {
-    bx -= nb - j;
-    ++j;
-  }
-  if (j >= e) return;
   const int i = j + bx;
-  const int c1 = (k + 1) * TILE;
-  const FactorOut o{P.L, P.Np, P.dinv + (size_t)(k + 1) * TILE * TILE,
+  if (i >= nb) return;
+  const int c1 = j * TILE;
+  const FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
                     P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr,
                     P.info};
-  syrk_tile<T, BF16, FUSE>((T*)P.L, P.Np, (int64_t)k * TILE, (int64_t)i * TILE,
-                           (int64_t)j * TILE, FUSE && i == k + 1 && j == k + 1, o);
+  syrk_tile_multi<T, BF16, FUSE>((T*)P.L, P.Np, s0, j, (int64_t)i * TILE, (int64_t)j * TILE,
+                                 FUSE && i == j, o);
 }
 
 // Outer update closing a super-step [s0, s_end): every lower tile (i, j),
@@ -771,12 +768,14 @@ static void column_launch(const Batch& b, int j, int s0, bool tpa, hipStream_t s
 
 // ---------------------------------------------------------------------------
 // Per block column k (every member at once), in super-steps of G columns:
-//   separate kernels: P_k panel; I_k inner update of the remaining columns of
-//     k's super-step (or, at its last column, O the outer update of
-//     everything right of it); F_k+1 factor the next diagonal block (inside
-//     I_k / O when that launch is small);
-//   column kernel: one k_column launch per column (left-looking update,
-//     factor, panel), then O at the super-step's end.
+//   separate kernels: U_k left-looking update of column k by the super-step's
+//     earlier panels; F_k factor of block k (inside U_k when that launch is
+//     small); P_k panel; at the super-step's last column O, the outer update
+//     of everything right of it (factoring its first block when small);
+//   column kernel: one k_column launch per column (the same update, factor
+//     and panel), then O at the super-step's end.
+// Both forms apply the same operations in the same order to every tile, so a
+// member's factor does not depend on which form its batch used.
 template <typename T, int BF16>
 static void potrf_steps(const Batch& b, hipStream_t s) {
   const int nb = b.maxNp / TILE;
@@ -849,7 +848,38 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
       if (outer && s_end < nb) outer_launch(s0, s_end, false);
       continue;
     }
-    if (k + 1 >= nb) break;  // the last diagonal block is already factored
+    // U_k (k > s0): left-looking update of column k, factoring block k inside
+    // the launch when it is small, else F_k after it.  Block s0 of a
+    // super-step was factored at the end of the previous one (or by F_0).
+    if (k > s0) {
+      auto tiles = [&](int m) { return std::max(0, b.np(m) / TILE - k); };
+      const bool fused = total_units(tiles) <= fuse_max();
+      sched_launch(b, tiles, [&](Prob* const* dp, const Sched& sc, int n) {
+        if (tpa) timer_begin(TC_PANEL, s);
+        if (fused)
+          hipLaunchKernelGGL((k_col_update<T, BF16, 1>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, k,
+                             s0);
+        else
+          hipLaunchKernelGGL((k_col_update<T, BF16, 0>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, k,
+                             s0);
+        if (tpa)
+          timer_end(TC_PANEL, s,
+                    flops(sc, dp, [&](int nbm) {
+                      return nbm > k ? (double)(nbm - k) * G64 * (k - s0) + (fused ? F3 : 0.0)
+                                     : 0.0;
+                    }),
+                    0.0);
+      });
+      if (!fused)
+        sched_launch(
+            b, [&](int m) { return b.np(m) / TILE > k ? 1 : 0; },
+            [&](Prob* const* dp, const Sched& sc, int n) {
+              if (tpa) timer_begin(TC_PANEL, s);
+              hipLaunchKernelGGL(k_factor_diag<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k);
+              if (tpa) timer_end(TC_PANEL, s, n * F3, 0.0);
+            });
+    }
+    if (k + 1 >= nb) break;
     sched_launch(
         b, [&](int m) { return std::max(0, b.np(m) / TILE - k - 1); },
         [&](Prob* const* dp, const Sched& sc, int n) {
@@ -859,48 +889,25 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
             timer_end(TC_PANEL, s,
                       flops(sc, dp, [&](int nbm) { return std::max(0, nbm - k - 1) * G64; }), 0.0);
         });
-    // Small update launches carry the next diagonal factorisation in the
-    // workgroup of tile (k+1, k+1): one launch fewer on the critical chain.
-    // Large ones keep the lean kernel and a separate factor launch.  If the
-    // next super-step uses the column kernel, that kernel factors block k+1.
-    const bool next_col = outer && col_style(s_end);
-    bool fused = false;
-    if (!outer) {
-      auto inner_tiles = [&](int nbm) {
-        int t = 0;
-        for (int j = k + 1; j < std::min(s_end, nbm); ++j) t += nbm - j;
-        return t;
-      };
-      fused = total_units([&](int m) { return inner_tiles(b.np(m) / TILE); }) <= fuse_max();
-      sched_launch(
-          b, [&](int m) { return inner_tiles(b.np(m) / TILE); },
-          [&](Prob* const* dp, const Sched& sc, int n) {
-            if (tpa) timer_begin(TC_PANEL, s);
-            if (fused)
-              hipLaunchKernelGGL((k_syrk_inner<T, BF16, 1>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
-                                 k, s_end);
-            else
-              hipLaunchKernelGGL((k_syrk_inner<T, BF16, 0>), dim3(n), dim3(NTHREADS), 0, s, dp, sc,
-                                 k, s_end);
-            if (tpa)
-              timer_end(TC_PANEL, s, flops(sc, dp, [&](int nbm) { return inner_tiles(nbm) * G64; }),
-                        0.0);
-          });
-    } else {
-      fused = !next_col && total_units([&](int m) {
-                             const int t = std::max(0, b.np(m) / TILE - s_end);
-                             return t * (t + 1) / 2;
-                           }) <= fuse_max();
+    if (outer) {
+      // O closes the super-step; block s_end is factored inside it when the
+      // launch is small, else by F_s_end -- unless the next super-step uses
+      // the column kernel, which factors it itself
+      const bool next_col = col_style(s_end);
+      const bool fused = !next_col && total_units([&](int m) {
+                                        const int t = std::max(0, b.np(m) / TILE - s_end);
+                                        return t * (t + 1) / 2;
+                                      }) <= fuse_max();
       outer_launch(s0, s_end, fused);
+      if (!fused && !next_col)
+        sched_launch(
+            b, [&](int m) { return b.np(m) / TILE > s_end ? 1 : 0; },
+            [&](Prob* const* dp, const Sched& sc, int n) {
+              if (tpa) timer_begin(TC_PANEL, s);
+              hipLaunchKernelGGL(k_factor_diag<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, s_end);
+              if (tpa) timer_end(TC_PANEL, s, n * F3, 0.0);
+            });
     }
-    if (!fused && !next_col)
-      sched_launch(
-          b, [&](int m) { return b.np(m) / TILE > k + 1 ? 1 : 0; },
-          [&](Prob* const* dp, const Sched& sc, int n) {
-            if (tpa) timer_begin(TC_PANEL, s);
-            hipLaunchKernelGGL(k_factor_diag<T>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k + 1);
-            if (tpa) timer_end(TC_PANEL, s, n * F3, 0.0);
-          });
   }
 }
 

@@ -201,6 +201,38 @@ def test_serving_alone_equals_in_mix():
     assert np.array_equal(a[2].var.cpu().numpy(), b[2].var.cpu().numpy())
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_serving_schedule_independent(mode):
+    """A member's results do not depend on the factorisation schedule its
+    batch gets: alone, the N = 1024 request is factored by the one-launch-per-
+    column kernel throughout; next to fifteen N = 2560 requests its first
+    super-steps go through the separate update / factor / panel kernels."""
+    from probit_amd.serving import ServingEngine
+    target = small_problem(90, 1024, 2, "probit", "EQ")
+    Xt = np.random.default_rng(91).normal(size=(64, 2))
+    others = [small_problem(100 + i, 2560, 2, "gaussian", "Matern12") for i in range(15)]
+
+    def run(with_others):
+        eng = ServingEngine(precision=mode, capacity=16, hbm_budget_bytes=1 << 34)
+        X, y, theta, lp, _ = target
+        assert eng.submit(0, X, y, "EQ", theta[0], theta[1], "probit", lp[0], lp[1], Xt)
+        if with_others:
+            for i, (Xo, yo, th, lpo, _) in enumerate(others):
+                assert eng.submit(1 + i, Xo, yo, "Matern12", th[0], th[1], "gaussian", lpo[0],
+                                  None, Xt)
+        out = {}
+        while eng.in_flight:
+            for c in eng.step():
+                out[c.tag] = c
+        eng.close()
+        return out[0]
+
+    a, b = run(False), run(True)
+    assert a.iterations == b.iterations
+    for k in ("mean", "var", "weight", "precision"):
+        assert np.array_equal(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy()), k
+
+
 def test_serving_async_launch_finish_matches_step():
     """The asynchronous launch / finish pair gives bit-identical results to
     the synchronous step (items are reported one step later)."""
