@@ -286,7 +286,10 @@ typedef struct pgp_batch_item {
  * step shared by all of them, then finalises and predicts the items that
  * converged and returns their tags.  Items join and leave between steps.
  * An item's results do not depend on which other items share its steps.
- * This is the batched form of LaplaceGP.approximate_posterior + predict
+ * The finalisation and predict of converged items run on a second stream
+ * that the batcher creates (default priority) and joins to `stream` within
+ * each step, so work the caller enqueues on `stream` afterwards sees the
+ * whole step.  This is the batched form of LaplaceGP.approximate_posterior + predict
  * (probit/approximators.py:204-210, 154-180). */
 typedef struct pgp_batcher pgp_batcher_t;
 int64_t pgp_batcher_table_size(int32_t capacity);

@@ -874,13 +874,18 @@ struct BItem {
 
 struct pgp_batcher {
   hipStream_t s;
+  // finalisation and predict of converged members run on their own stream,
+  // overlapping the Newton step of the others (disjoint members)
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   pgp_fit_options_t opts;
   int refine;
   int cap;
   Prob** arr[pgp::BATCH_ARRAYS];
   double* gather;      // per-member step norms (device)
   double* gather_fin;  // 8 doubles per finalised member (device)
-  uint32_t* ticket;
+  uint32_t* ticket;   // start-order ticket of launches on s
+  uint32_t* ticket2;  // ... and on s2 (separate cache line)
   Prob* ddesc;            // device staging of descriptors (2 * cap)
   Prob* hdesc = nullptr;  // pinned host staging of descriptors (2 * cap)
   // asynchronous launch / finish
@@ -908,20 +913,22 @@ static int64_t batcher_table_bytes(int cap) {
 // the half of the staging buffers, so that two writes of one step do not
 // overlap; `arr` is the batch's device pointer array (already written).
 static void write_probs(pgp_batcher* bt, const std::vector<BItem*>& v, Prob* const* arr,
-                        int region, int reset) {
+                        int region, int reset, hipStream_t s) {
   if (v.empty()) return;
   Prob* h = bt->hdesc + (size_t)region * bt->cap;
   Prob* d = bt->ddesc + (size_t)region * bt->cap;
   for (size_t i = 0; i < v.size(); ++i) h[i] = v[i]->p;
-  (void)hipMemcpyAsync(d, h, v.size() * sizeof(Prob), hipMemcpyHostToDevice, bt->s);
-  hipLaunchKernelGGL(k_scatter_probs, dim3((unsigned)v.size()), dim3(256), 0, bt->s, arr, d,
+  (void)hipMemcpyAsync(d, h, v.size() * sizeof(Prob), hipMemcpyHostToDevice, s);
+  hipLaunchKernelGGL(k_scatter_probs, dim3((unsigned)v.size()), dim3(256), 0, s, arr, d,
                      (int)v.size(), reset);
 }
 
 
 // Members are ordered by padded size, largest first (stable), so that the
 // members still active at any block column form a prefix of the batch.
+// Arrays 3..5 belong to the finalisation stream s2, the others to s.
 static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
+  const bool fin = which >= 3;
   std::stable_sort(v.begin(), v.end(),
                    [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
   std::vector<Prob*> ptrs;
@@ -937,14 +944,14 @@ static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
     hm.push_back(b->p.M);
     maxNp = std::max(maxNp, (int)b->p.Np);
   }
-  write_ptrs(bt->arr[which], ptrs, bt->s);
+  write_ptrs(bt->arr[which], ptrs, fin ? bt->s2 : bt->s);
   Batch b;
   b.dprobs = bt->arr[which];
   b.count = (int)v.size();
   b.maxNp = maxNp;
   b.hnp = hnp.data();
   b.hm = hm.data();
-  b.ticket = bt->ticket;
+  b.ticket = fin ? bt->ticket2 : bt->ticket;
   return b;
 }
 
@@ -963,17 +970,18 @@ static void admit(pgp_batcher* bt) {
     if (b.phase == 0) fresh.push_back(&b);
   if (!fresh.empty()) {
     const Batch bn = make_batch(bt, 0, fresh);
-    write_probs(bt, fresh, bn.dprobs, 0, 1);
+    write_probs(bt, fresh, bn.dprobs, 0, 1, s);
     launch_gram_self(bn, s);
     launch_vec_op(bn, OP_ZERO_Z, s);
     for (BItem* b : fresh) b->phase = 1;
   }
 }
 
-// Enqueue the final state (f, W, log p, B factored at W) and the predict of
-// the members in `conv`; 8 result doubles per member land in `gather`.
+// Enqueue on s2 the final state (f, W, log p, B factored at W) and the
+// predict of the members in `conv`; 8 result doubles per member land in
+// `gather` and are copied to the pinned hfin.
 static void finalize_and_predict(pgp_batcher* bt, std::vector<BItem*>& conv, double* gather) {
-  hipStream_t s = bt->s;
+  hipStream_t s = bt->s2;
   const int prec = bt->opts.precision;
   std::vector<BItem*> conv_fac, pred;
   for (BItem* b : conv)
@@ -999,12 +1007,13 @@ static void finalize_and_predict(pgp_batcher* bt, std::vector<BItem*>& conv, dou
   }
   if (!pred.empty()) {
     const Batch bp = make_batch(bt, 5, pred);
-    write_probs(bt, pred, bp.dprobs, 1, 0);
+    write_probs(bt, pred, bp.dprobs, 1, 0, s);
     const int maxMp = (int)round_up(maxM, TILE);
     launch_cross_prep(bp, prec, maxM, s);
     launch_trsm(bp, prec, maxMp, 0, s);
     launch_predict_final(bp, prec, maxM, s);
   }
+  (void)hipMemcpyAsync(bt->hfin, gather, conv.size() * 64, hipMemcpyDeviceToHost, s);
 }
 
 // Results of finalised members from their 8 gathered doubles each.
@@ -1068,6 +1077,7 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
                             align256((int64_t)capacity * 8 * 8));
   b->ticket = (uint32_t*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
                           2 * align256((int64_t)capacity * 8 * 8));
+  b->ticket2 = b->ticket + 32;
   b->ddesc = (Prob*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
                      2 * align256((int64_t)capacity * 8 * 8) + 256);
   b->hg.resize((size_t)capacity * 8);
@@ -1076,9 +1086,13 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
       hipHostMalloc((void**)&b->hfin, (size_t)capacity * 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&b->hdesc, (size_t)2 * capacity * sizeof(Prob),
                     hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamSynchronize(b->s) != hipSuccess) {
     pgp_batcher_destroy(b);
-    return fail(-100, "batcher set-up (table clear, pinned buffers, event) failed");
+    return fail(-100, "batcher set-up (table clear, pinned buffers, events, stream) failed");
   }
   *out = b;
   return 0;
@@ -1148,11 +1162,11 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
     if (converged(bt, b)) conv.push_back(b);
   }
   if (conv.empty()) return 0;
+  // s is idle (synchronised above), so s2 starts after the Newton step
   finalize_and_predict(bt, conv, bt->gather_fin);
   rc = check_launch();
   if (rc) return rc;
-  HIP_OK(hipMemcpyAsync(bt->hfin, bt->gather_fin, conv.size() * 64, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
+  HIP_OK(hipStreamSynchronize(bt->s2));
   report(bt, conv, bt->hfin, done_tags, max_done, n_done);
   drop_done(bt);
   return 0;
@@ -1175,10 +1189,14 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
       if (!b.gauss || b.iter == 0) fac.push_back(&b);
     }
   }
-  if (!bt->fin_list.empty()) {
+  const bool fork = !bt->fin_list.empty();
+  if (fork) {
+    // s2 follows everything enqueued on s so far, then runs beside the
+    // Newton step below; s joins it before the completion event
+    HIP_OK(hipEventRecord(bt->ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(bt->s2, bt->ev_fork, 0));
     finalize_and_predict(bt, bt->fin_list, bt->gather_fin);
-    HIP_OK(hipMemcpyAsync(bt->hfin, bt->gather_fin, bt->fin_list.size() * 64,
-                          hipMemcpyDeviceToHost, s));
+    HIP_OK(hipEventRecord(bt->ev_join, bt->s2));
     for (BItem* b : bt->fin_list) b->phase = 3;
   }
   if (!bt->act_list.empty()) {
@@ -1190,6 +1208,7 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
   }
   int rc = check_launch();
   if (rc) return rc;
+  if (fork) HIP_OK(hipStreamWaitEvent(s, bt->ev_join, 0));
   HIP_OK(hipEventRecord(bt->ev, s));
   bt->launched = true;
   return 0;
@@ -1219,7 +1238,13 @@ int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
 int pgp_batcher_destroy(pgp_batcher_t* bt) {
   if (!bt) return 0;
   if (bt->launched) (void)hipEventSynchronize(bt->ev);
+  if (bt->s2) {
+    (void)hipStreamSynchronize(bt->s2);
+    (void)hipStreamDestroy(bt->s2);
+  }
   if (bt->ev) (void)hipEventDestroy(bt->ev);
+  if (bt->ev_fork) (void)hipEventDestroy(bt->ev_fork);
+  if (bt->ev_join) (void)hipEventDestroy(bt->ev_join);
   if (bt->hstep) (void)hipHostFree(bt->hstep);
   if (bt->hfin) (void)hipHostFree(bt->hfin);
   if (bt->hdesc) (void)hipHostFree(bt->hdesc);
