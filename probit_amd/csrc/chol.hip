@@ -5,19 +5,19 @@
 //
 // Schedule for every batch member at once (c0 = 64k), with the block
 // columns grouped into super-steps of G columns:
-//   F    : factor diagonal block 0 (one workgroup per member);
-//   P_k  : panel  L_ik = A_ik L_kk^-T for all row blocks i > k, as an FP64
-//          MFMA product with the stored inverse of L_kk;
-//   I_k  : inner update A_ij -= L_ik L_jk^T of the later columns j of k's
-//          super-step (i >= j), 64x64 tiles on MFMA;
+//   U_k  : left-looking update A_ik -= sum_c L_ic L_kc^T of column k (i >= k)
+//          by the earlier columns c of k's super-step, 64x64 MFMA tiles;
+//   F_k  : factor and invert diagonal block k (one workgroup per member);
+//   P_k  : panel  L_ik = A_ik L_kk^-T for all row blocks i > k, as an MFMA
+//          product with the stored inverse of L_kk;
 //   O    : after the super-step's last column, the outer update of every
 //          lower tile right of it with rank 64 G (FP64 / FP32 operands, or
-//          BF16 operands with FP32 accumulation);
-//   F_k+1: factor and invert the next diagonal block.
+//          BF16 operands with FP32 accumulation).
 // A diagonal block is factored by one workgroup in FP64 together with its
-// inverse (blocked by 16: each 16x16 diagonal sub-block inside one wave, the
-// sub-panels and updates by the whole workgroup); the FP64 inverses (DINV)
-// serve the panel products and the triangular solves.
+// inverse (blocked by 16: each 16-column panel by one wave, one row per
+// lane; the trailing updates and the inverse's off-diagonal blocks on FP64
+// MFMA); the FP64 inverses (DINV) serve the panel products and the
+// triangular solves.
 #include <math.h>
 #include <stdlib.h>
 
@@ -78,47 +78,53 @@ __device__ __forceinline__ void mma16(const double (*A)[TILE + 1], int ar, int a
   }
 }
 
-// Factor the 16x16 diagonal sub-block at (o, o) of A in place (lower part)
-// and write its inverse to X(o.., o..) (zero above the diagonal).  One wave;
-// lane i (mod 16) holds row i in registers.  Each pivot goes through scalar
-// registers (readlane) and each scaled column through LDS, read back at one
-// address per step (a broadcast), so no block barrier is involved.
-__device__ __forceinline__ void factor16(double (*A)[TILE + 1], double (*X)[TILE + 1], int o,
-                                         double* dval, double* rinv, double* col,
-                                         int32_t* info) {
-  const int lane = threadIdx.x & 63;
-  const int i = lane & 15;
+// Factor the 16-column panel A(o.., o..o+15) of a 64x64 block in place: one
+// wave, lane l holds row o + l (16 values in registers; lanes past the
+// block's last row idle).  Values another row needs travel through scalar
+// registers (readlane), never through LDS; the pivot chain stays short
+// because row j+1's diagonal entry only needs its own row.  The rows below
+// the diagonal sub-block are finished here too, so no inverse is needed on
+// the way.  dval / rinv receive L(o+j, o+j) and its reciprocal square root.
+__device__ __forceinline__ void panel16(double (*A)[TILE + 1], int o, double* dval,
+                                        double* rinv, int32_t* info) {
+  const int l = threadIdx.x & 63;
+  const bool valid = l < TILE - o;
   double a[16];
 #pragma unroll
-  for (int m = 0; m < 16; ++m) a[m] = A[o + i][o + m];
+  for (int m = 0; m < 16; ++m) a[m] = valid ? A[o + l][o + m] : 0.0;
+  double myri = 0.0;
   bool bad = false;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const double p = bcast(a[j], j);
     const double ri = rsqrt_f64(p);
-    const double d = p * ri;
     bad |= !(p > 0.0);
-    const double lj = (i > j) ? a[j] * ri : (i == j ? d : 0.0);
-    if (i >= j) a[j] = lj;
-    col[16 * (j & 1) + i] = lj;  // all lanes of row i write the same value
-    __builtin_amdgcn_wave_barrier();
+    myri = (l == j) ? ri : myri;
+    const double lj = (l > j) ? a[j] * ri : (l == j ? p * ri : 0.0);
+    a[j] = lj;
+    // a(l, m) -= l(l, j) l(m, j) for the later columns m, the next pivot's
+    // column first
 #pragma unroll
-    for (int m = j + 1; m < 16; ++m) a[m] -= lj * col[16 * (j & 1) + m];
-    if (lane == 0) {
-      dval[o + j] = d;
-      rinv[j] = ri;
-    }
+    for (int m = j + 1; m < 16; ++m) a[m] -= lj * bcast(lj, m);
   }
-  if (lane == 0 && bad && info) atomicOr(info, PGP_INFO_NOT_PD);
-  if (lane < 16) {
+  if (l == 0 && bad && info) atomicOr(info, PGP_INFO_NOT_PD);
+  if (valid) {
 #pragma unroll
     for (int m = 0; m < 16; ++m)
-      if (m <= i) A[o + i][o + m] = a[m];
+      if (m <= l) A[o + l][o + m] = a[m];
   }
-  __builtin_amdgcn_wave_barrier();
-  // inverse by columns: lane c runs forward substitution on column c of I;
-  // L entries are read back from LDS (one address per step: a broadcast)
-  const int c = i;
+  if (l < 16) {
+    dval[o + l] = a[l];
+    rinv[o + l] = myri;
+  }
+}
+
+// X(o.., o..) = inverse of the factored 16x16 diagonal sub-block at (o, o)
+// (zero above the diagonal).  One wave; lane c runs forward substitution on
+// column c of I, reading L from LDS (one address per step: a broadcast).
+__device__ __forceinline__ void invert16(const double (*A)[TILE + 1], double (*X)[TILE + 1],
+                                         int o, const double* rinv) {
+  const int c = threadIdx.x & 15;
   double x[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -129,41 +135,34 @@ __device__ __forceinline__ void factor16(double (*A)[TILE + 1], double (*X)[TILE
       acc1 -= A[o + r][o + k + 1] * x[k + 1];
     }
     if (r & 1) acc0 -= A[o + r][o + r - 1] * x[r - 1];
-    x[r] = (acc0 + acc1) * rinv[r];
+    x[r] = (acc0 + acc1) * rinv[o + r];
   }
-  if (lane < 16) {
+  if ((threadIdx.x & 63) < 16) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) X[o + r][o + c] = (r >= c) ? x[r] : 0.0;
   }
 }
 
 // Blocked factor + inverse of a 64x64 SPD block held in LDS (`A`, [row][65]).
-// Per 16-wide sub-panel s: one wave factors (and inverts) the diagonal
-// sub-block, then FP64 MFMA products form the sub-panel L_rs = A_rs X_ss^T
-// and the trailing update A_rc -= L_rs L_cs^T.  The off-diagonal blocks of
+// Per 16-wide panel s: one wave factors the panel (panel16), then FP64 MFMA
+// products apply it to the trailing blocks, A_rc -= L_rs L_cs^T.  Then each
+// wave inverts one diagonal sub-block, and the off-diagonal blocks of
 // X = L^-1 follow by block distance: X_rs = -X_rr sum_{k=s}^{r-1} L_rk X_ks.
 // L (lower) goes to global storage as T, X to o.dinv (FP64).
 template <typename T>
 __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], double (*X)[TILE + 1], int64_t r0,
                                      const FactorOut& o) {
-  __shared__ double dval[TILE], rinv[16], col[32];
+  __shared__ double dval[TILE], rinv[TILE];
   const int t = threadIdx.x;
   const int wv = t >> 6, l = t & 63;
   const int orow = l >> 4, ocol = l & 15;  // D layout: row orow + 4r, col ocol
   __builtin_amdgcn_s_setprio(3);
   for (int s = 0; s < 4; ++s) {
     const int o16 = 16 * s;
-    if (wv == 0) factor16(A, X, o16, dval, rinv, col, o.info);
+    if (wv == 0) panel16(A, o16, dval, rinv, o.info);
     __syncthreads();
+    if (s == 3) break;
     const int nrb = 3 - s;
-    if (wv < nrb) {  // sub-panel block rb: L_rb,s = A_rb,s X_ss^T
-      const int rb = s + 1 + wv;
-      doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-      mma16(A, 16 * rb, o16, X, o16, o16, 1, acc);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) A[16 * rb + orow + 4 * r][o16 + ocol] = acc[r];
-    }
-    __syncthreads();
     // trailing blocks (r, c), s < c <= r: A_rc -= L_rs L_cs^T
     for (int b = wv; b < nrb * (nrb + 1) / 2; b += 4) {
       int rr = 0;
@@ -177,6 +176,8 @@ __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], doub
     }
     __syncthreads();
   }
+  invert16(A, X, 16 * wv, rinv);
+  __syncthreads();
   // off-diagonal blocks of X by distance d; T goes to A's upper block (s, r)
   for (int d = 1; d < 4; ++d) {
     const int nblk = 4 - d;
