@@ -207,7 +207,8 @@ __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], doub
   for (int e = t; e < TILE * TILE; e += NTHREADS) {
     const int c = e / TILE, r = e % TILE;  // column-major walk: coalesced stores
     if (c <= r) L[(size_t)(r0 + r) + (size_t)(r0 + c) * o.ld] = (T)A[r][c];
-    o.dinv[r + c * TILE] = (c <= r) ? X[r][c] : 0.0;
+    // coherent: the column kernel hands DINV to other workgroups
+    st_coherent(&o.dinv[r + c * TILE], (c <= r) ? X[r][c] : 0.0);
   }
 }
 
@@ -572,8 +573,8 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
                 P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
                 P.info};
     factor_block_blocked<T>(arena[0], arena[1], c0, o);
-    // every wave's stores complete and written back, then publish
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wave's DINV stores complete, then publish
+    publish_wait();
     __syncthreads();
     if (threadIdx.x == 0)
       __hip_atomic_store(P.progress, ((uint64_t)epoch << 32) | 1u, __ATOMIC_RELAXED,
@@ -590,14 +591,13 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
   update_tile_to_lds<T, BF16, T>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena, ck, PP, true);
   if (threadIdx.x == 0) wait_progress(P.progress, epoch, 1, P.info);
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  // B[p][n] = DINV_j(n, p)
+  // B[p][n] = DINV_j(n, p), published in this launch: coherent loads
   const double* Dj = P.dinv + (size_t)j * TILE * TILE;
   {
     const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
     for (int q = 0; q < 16; ++q) {
       const int pp = pb + 4 * q;
-      bk[pp * PP + n] = (T)Dj[n + pp * TILE];
+      bk[pp * PP + n] = (T)ld_coherent(&Dj[n + pp * TILE]);
     }
   }
   __syncthreads();

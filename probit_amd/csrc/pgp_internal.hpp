@@ -150,14 +150,22 @@ inline void sched_launch(const Batch& b, Units units, Go go) {
 constexpr int SPIN_LIMIT = 1 << 22;
 uint32_t next_epoch();  // fresh epoch per launch (never 0)
 
-// Relaxed agent-scope (L2-coherent) accesses for values handed between
-// workgroups of one launch.
+// Values handed between workgroups of one launch travel only through
+// relaxed agent-scope accesses: the stores write through the XCD's L2 and
+// the loads read past it, so a publication needs no L2 write-back or
+// invalidation (the agent-scope fences, which act on the whole L2 of the
+// XCD and slow every workgroup there).  A writer makes its coherent stores
+// complete (publish_wait) before it stores the progress word; a reader
+// issues its coherent loads after it has seen the word.
 __device__ __forceinline__ double ld_coherent(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_coherent(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// This wave's outstanding global stores (and loads) complete; also a
+// compiler barrier for memory.
+__device__ __forceinline__ void publish_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Ticket of this workgroup in start order (thread 0 takes it; the last
 // ticket resets the counter for the next launch on the stream).
@@ -188,7 +196,8 @@ __device__ __forceinline__ int wait_progress(uint64_t* progress, uint32_t epoch,
     }
     __builtin_amdgcn_s_sleep(1);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // the published data is read with ld_coherent, issued after this point
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   return r;
 }
 

@@ -154,8 +154,8 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sche
     if (done >= ready) {
       __syncthreads();  // everyone has read the previous s_val
       if (threadIdx.x == 0) {
-        // poll with relaxed loads (no cache maintenance per poll), then one
-        // acquire fence once progress is seen
+        // poll with relaxed loads; the published x is read with coherent
+        // loads issued after the poll, so no cache maintenance is needed
         int r = 0;
         for (int spins = 0;; ++spins) {
           const uint64_t pv = __hip_atomic_load(P.progress, __ATOMIC_RELAXED,
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sche
           }
           __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         s_val = r;
       }
       __syncthreads();
@@ -252,9 +252,9 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sche
     a += (trans ? dk[q + l * (TILE + 1)] : dk[l + q * (TILE + 1)]) * v[q];
   part[w][l] = a;
   __syncthreads();
-  if (threadIdx.x < TILE) {  // wave 0: store x_i, one release fence, then publish
+  if (threadIdx.x < TILE) {  // wave 0: store x_i, wait for the stores, then publish
     st_coherent(&x[ri + l], (part[0][l] + part[1][l]) + (part[2][l] + part[3][l]));
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    publish_wait();
     if (threadIdx.x == 0)
       __hip_atomic_store(P.progress, ((uint64_t)epoch << 32) | (uint64_t)(local + 1),
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
