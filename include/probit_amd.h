@@ -232,6 +232,17 @@ int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M,
                 int64_t workspace_bytes, void* predict_workspace,
                 int64_t predict_workspace_bytes, void* stream);
 
+/* Posterior predictive covariance at M test points from the resident factor
+ * (replaces Approximator.predict_covariance, probit/approximators.py:182-197):
+ * cov = Kss - Kfs^T (K + W^-1)^-1 Kfs, column-major M x M FP64 with leading
+ * dimension ld >= M, computed as Kss - V^T V with V = L_B^-1 W^1/2 Kfs (the
+ * same triangular solve as pgp_predict; FP64 MFMA product) and written
+ * exactly symmetric.  Workspaces as for pgp_predict. */
+int pgp_predict_covariance(const pgp_kernel_t* kernel, const double* Xtest,
+                           int64_t M, double* cov, int64_t ld, void* workspace,
+                           int64_t workspace_bytes, void* predict_workspace,
+                           int64_t predict_workspace_bytes, void* stream);
+
 /* Gradient workspace bytes. */
 int64_t pgp_grad_workspace_size(int64_t N, int32_t D);
 

@@ -143,6 +143,8 @@ _SIGNATURES = {
                                         C.POINTER(LikelihoodT), _I32, _P, _P,
                                         C.POINTER(FitResultT), _P, _I64, _P]),
     "pgp_predict": (C.c_int, [C.POINTER(KernelT), _P, _I64, _P, _P, _P, _I64, _P, _I64, _P]),
+    "pgp_predict_covariance": (C.c_int, [C.POINTER(KernelT), _P, _I64, _P, _I64, _P, _I64, _P,
+                                         _I64, _P]),
     "pgp_laplace_grad": (C.c_int, [C.POINTER(KernelT), C.POINTER(LikelihoodT), _P, _P, _I64, _P,
                                    _I64, _P]),
     "pgp_vb_fit": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT), C.POINTER(LikelihoodT),

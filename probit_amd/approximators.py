@@ -211,8 +211,9 @@ class Approximator(ABC):
         return w, Wt, fout, res
 
     # ---- public API ---------------------------------------------------------
-    def predict(self, X_test, parameters, weight, precision):
-        """Latent predictive mean and variance (probit/approximators.py:154-180)."""
+    def _predict_setup(self, X_test, parameters, weight, precision):
+        """Make the resident factor match (parameters, weight, precision) and
+        stage X_test (M x D FP64 on the device) and the predict workspace."""
         lib = _lib.lib()
         ks, ls = self._structs(parameters)
         key = self._key(ks, ls)
@@ -232,6 +233,11 @@ class Approximator(ABC):
         need = lib.pgp_predict_workspace_size(self.N, M, self._prec)
         if self._pws is None or self._pws.numel() < need:
             self._pws = _lib.workspace(need, self.device)
+        return lib, ks, Xt, M
+
+    def predict(self, X_test, parameters, weight, precision):
+        """Latent predictive mean and variance (probit/approximators.py:154-180)."""
+        lib, ks, Xt, M = self._predict_setup(X_test, parameters, weight, precision)
         mean = torch.empty(M, dtype=torch.float64, device=self.device)
         var = torch.empty(M, dtype=torch.float64, device=self.device)
         ws = self._workspace()
@@ -242,8 +248,18 @@ class Approximator(ABC):
         return mean, var
 
     def predict_covariance(self, X_test, parameters, weight, precision):
-        """Predictive covariance Kss - Kfs^T (K + W^-1)^-1 Kfs (probit/approximators.py:182-197)."""
-        raise NotImplementedError("predict_covariance is not provided by this build")
+        """Predictive covariance Kss - Kfs^T (K + W^-1)^-1 Kfs, an (M, M) tensor
+        (probit/approximators.py:182-197)."""
+        lib, ks, Xt, M = self._predict_setup(X_test, parameters, weight, precision)
+        cov = torch.empty((M, M), dtype=torch.float64, device=self.device)
+        ws = self._workspace()
+        # column-major with ld = M is the transpose of torch's row-major
+        # layout; the result is symmetric, so the same memory serves
+        rc = lib.pgp_predict_covariance(ks, Xt.data_ptr(), M, cov.data_ptr(), M, ws.data_ptr(),
+                                        ws.numel(), self._pws.data_ptr(), self._pws.numel(),
+                                        self._stream())
+        _lib.check(rc, "pgp_predict_covariance")
+        return cov
 
     def posterior_mean(self, weight):
         """The reference's posterior_mean (probit/approximators.py:199-202) reads an

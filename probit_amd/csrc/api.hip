@@ -682,6 +682,42 @@ int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M, doub
   return check_launch();
 }
 
+int pgp_predict_covariance(const pgp_kernel_t* kernel, const double* Xtest, int64_t M,
+                           double* cov, int64_t ld, void* workspace, int64_t workspace_bytes,
+                           void* predict_workspace, int64_t predict_workspace_bytes,
+                           void* stream) {
+  (void)kernel;
+  if (!Xtest || M <= 0) return fail(-2, "Xtest / M invalid");
+  if (!cov) return fail(-4, "cov is NULL");
+  if (ld < M) return fail(-5, "ld < M");
+  if (!workspace) return fail(-6, "workspace is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  Prob p;
+  int rc = read_prob(ws, p, s);
+  if (rc) return rc;
+  const WsLayout L = ws_layout(p.N, p.prec);
+  if (workspace_bytes < L.total) return fail(-7, "workspace too small");
+  const PredLayout PL = pred_layout(p.Np, M, p.prec);
+  if (!predict_workspace || predict_workspace_bytes < PL.total)
+    return fail(-9, "predict workspace too small: %lld < %lld",
+                (long long)predict_workspace_bytes, (long long)PL.total);
+  char* pw = (char*)predict_workspace;
+  p.Xt = Xtest;
+  p.M = (int32_t)M;
+  p.Mp = (int32_t)PL.Mp;
+  p.S = pw + PL.S;
+  p.mupart = (double*)(pw + PL.mupart);
+  p.pmean = nullptr;
+  p.pvar = nullptr;
+  write_prob(ws, p, 0, s);
+  const Batch b = single_batch(ws, p.Np);
+  launch_cross_prep(b, p.prec, (int)M, s);
+  launch_trsm(b, p.prec, (int)PL.Mp, 0, s);
+  launch_pred_cov(b, p.prec, (int)PL.Mp, cov, ld, s);
+  return check_launch();
+}
+
 int pgp_laplace_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik, double* grad_out,
                      void* workspace, int64_t workspace_bytes, void* grad_workspace,
                      int64_t grad_workspace_bytes, void* stream) {

@@ -11,6 +11,7 @@
 // an LDS transpose, so HBM sees each entry written once.
 #include <math.h>
 
+#include "mfma.hpp"
 #include "pgp_internal.hpp"
 
 namespace pgp {
@@ -256,6 +257,89 @@ void launch_cross_prep(const Batch& b, int prec, int maxM, hipStream_t s) {
           hipLaunchKernelGGL(k_cross_prep<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
         else
           hipLaunchKernelGGL(k_cross_prep<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+      });
+}
+
+// ---- predictive covariance: cov = Kss - V^T V, V = L^-1 S after the TRSM ----
+// One 64x64 lower tile (ti >= tj) of the M x M output per workgroup: an FP64
+// MFMA product over the Np rows of V (FP32 values of V widen exactly), then
+// Kss minus it, written to the tile and its mirror (the diagonal tile from
+// its lower half), so the output is exactly symmetric.
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_pred_cov(Prob* const* probs, Sched sc, double* cov,
+                                                       int64_t ld) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int Mt = P.Mp / TILE;
+  if (bx >= Mt * (Mt + 1) / 2) return;
+  int ti, tj;
+  lower_tile(bx, ti, tj);
+  constexpr int PA = TILE + 1;
+  __shared__ double sa[TILE * PA], sb[TILE * PA];
+  __shared__ double xs[TILE * PGP_MAX_DIM];
+  __shared__ double zs[TILE * PGP_MAX_DIM];
+  __shared__ KShared ks;
+  const int D = P.D, M = P.M;
+  stage_params(ks, P.inv_ls2, P.pi_over_per);
+  load_rows(xs, P.Xt, ti * TILE, M, D);
+  load_rows(zs, P.Xt, tj * TILE, M, D);
+  const T* V = (const T*)P.S;
+  const size_t np = (size_t)P.Np;
+  const int kl = threadIdx.x & 63, cg = threadIdx.x >> 6;
+  doublex4 acc[2][2];
+  zero_acc<double>(acc);
+  // k-major operands sa[k][m] = V(k0 + k, 64 ti + m), read coalesced along
+  // k; the next chunk is fetched into registers while this one multiplies
+  T ra[16], rb[16];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cg + 4 * q;
+      ra[q] = V[(size_t)(k0 + kl) + (size_t)(ti * TILE + c) * np];
+      rb[q] = V[(size_t)(k0 + kl) + (size_t)(tj * TILE + c) * np];
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < P.Np; k0 += TILE) {
+    __syncthreads();  // the previous chunk's products are done with LDS
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cg + 4 * q;
+      sa[kl * PA + c] = (double)ra[q];
+      sb[kl * PA + c] = (double)rb[q];
+    }
+    __syncthreads();
+    if (k0 + TILE < P.Np) fetch(k0 + TILE);
+    mma64_kmajor<double>(sa, PA, sb, PA, acc);
+  }
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = acc_row<double>(bm, r), col = acc_col(bn);
+        const int i = ti * TILE + row, j = tj * TILE + col;
+        if (i >= M || j >= M || (ti == tj && row < col)) continue;
+        const double v =
+            kernel_value<double>(P.kern_kind, P.periodic, D, P.variance, ks, &xs[row * D],
+                                 &zs[col * D]) -
+            acc[bm][bn][r];
+        cov[(size_t)i + (size_t)j * ld] = v;
+        cov[(size_t)j + (size_t)i * ld] = v;
+      }
+}
+
+void launch_pred_cov(const Batch& b, int prec, int maxMp, double* cov, int64_t ld,
+                     hipStream_t s) {
+  const int mt = maxMp / TILE;
+  sched_launch(
+      b, [&](int) { return mt * (mt + 1) / 2; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        if (prec == PGP_PREC_FP64)
+          hipLaunchKernelGGL(k_pred_cov<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, cov, ld);
+        else
+          hipLaunchKernelGGL(k_pred_cov<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, cov, ld);
       });
 }
 

@@ -264,6 +264,8 @@ void launch_vec_op(const Batch& b, int op, hipStream_t s);
 void launch_copy_slot(const Batch& b, int src, int dst, hipStream_t s);
 void launch_reduce(const Batch& b, int what, double* gather, hipStream_t s);
 void launch_predict_final(const Batch& b, int prec, int maxM, hipStream_t s);
+void launch_pred_cov(const Batch& b, int prec, int maxMp, double* cov, int64_t ld,
+                     hipStream_t s);
 // weight -> Z (if non-NULL), W -> W and S = sqrt(max(W, 0)) (if non-NULL); zero padding
 void launch_load_state(const Batch& b, const double* weight, const double* W, hipStream_t s);
 

@@ -120,6 +120,28 @@ def test_predict_with_foreign_weight_and_precision():
     assert rel_err(var.cpu().numpy(), v_ref) < 1e-10
 
 
+@pytest.mark.parametrize("cls,lik,mode,M", [("LaplaceGP", "probit", "fp64", 100),
+                                            ("LaplaceGP", "gaussian", "fp64", 64),
+                                            ("LaplaceGP", "probit", "fp32", 130),
+                                            ("VBGP", "probit", "fp64", 70)])
+def test_predict_covariance_matches_reference(cls, lik, mode, M):
+    """predict_covariance (probit/approximators.py:182-197) against the oracle's
+    dense solve; its diagonal against predict's variance; exact symmetry."""
+    X, y, theta, lp, spec = small_problem(21, 150, 2, lik, "Matern12")
+    gp = _gp(cls, X, y, "Matern12", lik, precision_mode=mode)
+    params = (theta, lp)
+    w, p = gp.approximate_posterior(params)
+    Xt = np.random.default_rng(22).normal(size=(M, 2))
+    cov = gp.predict_covariance(Xt, params, w, p).cpu().numpy()
+    _, var = gp.predict(Xt, params, w, p)
+    ref = O.predict_covariance(spec, X, Xt, w.cpu().numpy(), p.cpu().numpy())
+    assert cov.shape == (M, M)
+    assert np.array_equal(cov, cov.T)
+    tol = 1e-10 if mode == "fp64" else 1e-5
+    assert np.max(np.abs(cov - ref)) <= tol * np.max(np.abs(ref))
+    assert np.max(np.abs(np.diag(cov) - var.cpu().numpy())) <= 1e-12 * float(theta[1])
+
+
 def test_precision_and_construct():
     X, y, theta, lp, spec = small_problem(13, 70, 2, "probit", "Matern12")
     gp = _gp("LaplaceGP", X, y, "Matern12", "probit")

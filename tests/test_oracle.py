@@ -246,6 +246,10 @@ def test_gaussian_predict_and_objective_match_sklearn(kind, per):
     assert np.allclose(var, s_sk ** 2, rtol=1e-7, atol=1e-10)
     obj = O.objective_LA(spec, X, y, O.GAUSSIAN, lp, w)
     assert np.isclose(obj, -gpr.log_marginal_likelihood_value_, rtol=1e-10)
+    cov = O.predict_covariance(spec, X, Xt, w, p)
+    _, cov_sk = gpr.predict(Xt, return_cov=True)
+    assert np.allclose(cov, cov_sk, rtol=1e-7, atol=1e-10)
+    assert np.allclose(np.diag(cov), var, rtol=1e-10, atol=1e-13)
 
 
 def test_gaussian_gradient_matches_sklearn():
