@@ -352,7 +352,13 @@ int launch_trsv_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const d
   return 0;
 }
 
-// ---- TRSM: S <- L^-1 S, S is Np x Mp (ld lds), one launch per block row ----
+// ---- TRSM: S <- L^-1 S, S is Np x Mp (ld lds) ----
+// Two schedules.  For narrow right-hand sides (a few column tiles, as in
+// serving) one launch per block row keeps every launch to two short tile
+// products per workgroup; for wide ones the super-step schedule below reads
+// and writes S G times less often.
+//
+// Per block row (lagged).
 // Lagged schedule.  Launch k, workgroup (w, ct), row block i = k + w:
 //   S_i[:, ct] -= L_{i,k-1} X_{k-1}[:, ct]   (k >= 1; X_{k-1} = S_{k-1}, final
 //                                            since launch k - 1)
@@ -428,8 +434,8 @@ __device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* 
 }
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_trsm(Prob* const* probs, Sched sc, int k,
-                                                   int lower_rhs) {
+__global__ __launch_bounds__(NTHREADS) void k_trsm_lagged(Prob* const* probs, Sched sc, int k,
+                                                          int lower_rhs) {
   int bx;
   const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int nb = P.Np / TILE;
@@ -440,7 +446,156 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm(Prob* const* probs, Sched sc,
   trsm_step<T>((const T*)P.L, P.Np, P.dinv, k, (T*)P.S, P.Np, w, ct, lower_rhs);
 }
 
-void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_t s) {
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_trsm_lagged_raw(const T* L, int64_t ld,
+                                                              const double* dinv, int k, T* S,
+                                                              int64_t lds, int lower_rhs) {
+  trsm_step<T>(L, ld, dinv, k, S, lds, blockIdx.x, blockIdx.y, lower_rhs);
+}
+
+// Super-steps of G block rows.
+// One launch per super-step q of block rows SS_q = [Gq, Gq + G), with
+// P = SS_{q-1} the super-step finished by the previous launch:
+//   solve workgroup (column tile ct), for k in SS_q in order:
+//     X_k = DINV_k (S_k - sum_{c in [G(q-1), k)} L_kc X_c)   (left-looking)
+//   update workgroups (row block i >= G(q+1), column tile ct):
+//     S_i -= sum_{c in P} L_ic X_c                          (rank 64 G)
+// so every later row block receives each finished super-step in one
+// rank-64G product: S is read and written once per super-step, not once per
+// block row.  A workgroup writes only its own rows and reads rows finished
+// in earlier launches (or by itself), so workgroups of one launch never
+// race.  lower_rhs: S is lower triangular (identity-like right-hand side):
+// the blocks X_c[:, ct] with c < ct are zero and are skipped.
+constexpr int TRSM_G = 4;
+
+// acc += sum_{c in [c0, c1)} L(r0.., 64c..) S(64c.., n0..): MFMA tile
+// products with the next chunk fetched into registers during the current.
+// Rows of S from block c_fresh on were written by this workgroup in this
+// launch: they are read with coherent loads (past the CU's L1, which may
+// still hold the values they replaced).
+template <typename T>
+__device__ void trsm_acc(const T* L, int64_t ld, const T* S, int64_t lds, int64_t r0, int64_t n0,
+                         int c0, int c1, int c_fresh, T* As, T* Bs,
+                         typename Acc<T>::type (&acc)[2][2]) {
+  constexpr int PP = KP<T>::value;  // A operand: conflict-free MFMA reads
+  constexpr int PB = TILE + 1;       // B operand: written transposed, conflict-free stores
+  if (c0 >= c1) return;
+  const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
+  T ra[16], rb[16];
+  auto fetch = [&](int c) {
+    const int64_t cc = (int64_t)c * TILE;
+    const bool fresh = c >= c_fresh;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = kb + 4 * q;
+      ra[q] = L[(size_t)(r0 + m) + (size_t)(cc + k) * ld];  // A(m, k)
+      const T* sp = &S[(size_t)(cc + m) + (size_t)(n0 + k) * lds];  // B(m, k): row m, column k
+      rb[q] = fresh ? __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *sp;
+    }
+  };
+  fetch(c0);
+  for (int c = c0; c < c1; ++c) {
+    if (c > c0) __syncthreads();  // the previous chunk's products are done with LDS
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = kb + 4 * q;
+      As[k * PP + m] = ra[q];
+      Bs[m * PB + k] = rb[q];
+    }
+    __syncthreads();
+    if (c + 1 < c1) fetch(c + 1);
+    mma64_kmajor<T>(As, PP, Bs, PB, acc);
+  }
+  __syncthreads();  // LDS free for the caller
+}
+
+// Workgroup `u` of launch q for one right-hand side of nct column tiles:
+// u < nct solves SS_q for column tile u; the others update one row block.
+template <typename T>
+__device__ void trsm_ss(const T* L, int64_t ld, const double* dinv, int nb, int q, T* S,
+                        int64_t lds, int nct, int u, int lower_rhs) {
+  constexpr int PP = KP<T>::value;
+  constexpr int PB = TILE + 1;
+  __shared__ T As[TILE * PP];
+  __shared__ T Bs[TILE * PB];
+  typedef typename Acc<T>::type acc_t;
+  constexpr int G = TRSM_G;
+  const int p0 = q > 0 ? G * (q - 1) : 0;  // first column of P (or of SS_0)
+  acc_t acc[2][2];
+  if (u < nct) {
+    const int ct = u;
+    const int64_t n0 = (int64_t)ct * TILE;
+    const int kend = min(G * q + G, nb);
+    for (int k = G * q; k < kend; ++k) {
+      if (lower_rhs && ct > k) continue;  // X_k[:, ct] stays zero
+      const int64_t r0 = (int64_t)k * TILE;
+      zero_acc<T>(acc);
+      trsm_acc<T>(L, ld, S, lds, r0, n0, lower_rhs ? max(p0, ct) : p0, k, G * q, As, Bs, acc);
+      // B = S_k - acc (rounded to T), A = DINV_k
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = acc_row<T>(bm, r), col = acc_col(bn);
+            Bs[row * PB + col] =
+                S[(size_t)(r0 + row) + (size_t)(n0 + col) * lds] - acc[bm][bn][r];
+          }
+      load_block_kmajor<double, T>(As, PP, dinv + (size_t)k * TILE * TILE, TILE, 0, 0);
+      __syncthreads();
+      zero_acc<T>(acc);
+      mma64_kmajor<T>(As, PP, Bs, PB, acc);
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            S[(size_t)(r0 + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds] =
+                acc[bm][bn][r];
+      publish_wait();   // X_k reaches L2 ...
+      __syncthreads();  // ... before the next row of this super-step reads it back
+    }
+    return;
+  }
+  // update of row block i >= G(q+1) by P = [G(q-1), Gq)
+  const int v = u - nct;
+  const int ct = v % nct;
+  const int i = G * (q + 1) + v / nct;
+  if (q == 0 || i >= nb) return;
+  const int c0 = lower_rhs ? max(p0, ct) : p0;
+  if (c0 >= G * q) return;
+  const int64_t r0 = (int64_t)i * TILE, n0 = (int64_t)ct * TILE;
+  zero_acc<T>(acc);
+  trsm_acc<T>(L, ld, S, lds, r0, n0, c0, G * q, G * q, As, Bs, acc);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T* p = &S[(size_t)(r0 + acc_row<T>(bm, r)) + (size_t)(n0 + acc_col(bn)) * lds];
+        *p = *p - acc[bm][bn][r];
+      }
+}
+
+// workgroups of launch q: nct solvers plus nct per row block below SS_q
+static inline int trsm_units(int nb, int q, int nct) {
+  const int rows = q > 0 ? std::max(0, nb - TRSM_G * (q + 1)) : 0;
+  return nb > TRSM_G * q ? nct * (1 + rows) : 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_trsm(Prob* const* probs, Sched sc, int q,
+                                                   int lower_rhs) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  trsm_ss<T>((const T*)P.L, P.Np, P.dinv, P.Np / TILE, q, (T*)P.S, P.Np, P.Mp / TILE, bx,
+             lower_rhs);
+}
+
+static void launch_trsm_lagged(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_t s) {
   const int nb = bt.maxNp / TILE;
   const bool on = timer_on(TC_TRSM);
   for (int k = 0; k < nb; ++k) {
@@ -459,9 +614,9 @@ void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_
         [&](Prob* const* dp, const Sched& sc, int n) {
           if (on) timer_begin(TC_TRSM, s);
           if (prec == PGP_PREC_FP64)
-            hipLaunchKernelGGL(k_trsm<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
+            hipLaunchKernelGGL(k_trsm_lagged<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
           else
-            hipLaunchKernelGGL(k_trsm<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
+            hipLaunchKernelGGL(k_trsm_lagged<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, k, lower_rhs);
           if (on) {
             // step k of an N^2 M-flop solve: the update of rows k.. by block
             // k - 1 (2 * 64 per entry) and the 64-row triangle (64^2 per column)
@@ -479,23 +634,84 @@ void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_
   }
 }
 
+// Right-hand sides of at least this many column tiles take the super-step
+// schedule.
+constexpr int TRSM_WIDE = 16;
+
+void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_t s) {
+  if (maxMp / TILE < TRSM_WIDE) {
+    launch_trsm_lagged(bt, prec, maxMp, lower_rhs, s);
+    return;
+  }
+  const int nb = bt.maxNp / TILE;
+  const int nq = (nb + TRSM_G - 1) / TRSM_G;
+  const bool on = timer_on(TC_TRSM);
+  auto nct_of = [&](int m) {
+    const int mp = bt.hm ? (bt.hm[m] + TILE - 1) / TILE * TILE : maxMp;
+    return mp / TILE;
+  };
+  for (int q = 0; q < nq; ++q) {
+    sched_launch(
+        bt, [&](int m) { return trsm_units(bt.np(m) / TILE, q, nct_of(m)); },
+        [&](Prob* const* dp, const Sched& sc, int n) {
+          if (on) timer_begin(TC_TRSM, s);
+          if (prec == PGP_PREC_FP64)
+            hipLaunchKernelGGL(k_trsm<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, q, lower_rhs);
+          else
+            hipLaunchKernelGGL(k_trsm<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, q, lower_rhs);
+          if (on) {
+            // super-step q of an N^2 M-flop solve: its rows' left-looking
+            // products and triangles, and the rank-64G update of the rows
+            // below by the previous super-step (full right-hand side)
+            constexpr double T3 = 2.0 * TILE * TILE * TILE;
+            double fl = 0.0;
+            const int off = (int)(dp - bt.dprobs);
+            for (int m = off; m < off + sc.count; ++m) {
+              const int nbm = bt.np(m) / TILE, nct = nct_of(m);
+              if (nbm <= TRSM_G * q) continue;
+              const int p0 = q > 0 ? TRSM_G * (q - 1) : 0;
+              for (int k = TRSM_G * q; k < std::min(nbm, TRSM_G * q + TRSM_G); ++k)
+                fl += nct * (T3 * (k - p0) + T3 / 2);
+              if (q > 0)
+                fl += (double)nct * std::max(0, nbm - TRSM_G * (q + 1)) * T3 * TRSM_G;
+            }
+            timer_end(TC_TRSM, s, fl, 0.0);
+          }
+        });
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void k_trsm_raw(const T* L, int64_t ld, const double* dinv,
-                                                       int k, T* S, int64_t lds, int lower_rhs) {
-  trsm_step<T>(L, ld, dinv, k, S, lds, blockIdx.x, blockIdx.y, lower_rhs);
+                                                       int nb, int q, T* S, int64_t lds, int nct,
+                                                       int lower_rhs) {
+  trsm_ss<T>(L, ld, dinv, nb, q, S, lds, nct, blockIdx.x, lower_rhs);
 }
 
 int launch_trsm_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const double* dinv,
                          void* S, int64_t M, int64_t lds, int lower_rhs, hipStream_t s) {
-  const int nb = (int)(N / TILE);
-  for (int k = 0; k < nb; ++k) {
-    dim3 g(k == 0 ? 1 : nb - k, (unsigned)(M / TILE));
+  const int nb = (int)(N / TILE), nct = (int)(M / TILE);
+  if (nct < TRSM_WIDE) {
+    for (int k = 0; k < nb; ++k) {
+      dim3 g(k == 0 ? 1 : nb - k, (unsigned)nct);
+      if (prec == PGP_PREC_FP64)
+        hipLaunchKernelGGL(k_trsm_lagged_raw<double>, g, dim3(NTHREADS), 0, s, (const double*)L,
+                           ld, dinv, k, (double*)S, lds, lower_rhs);
+      else
+        hipLaunchKernelGGL(k_trsm_lagged_raw<float>, g, dim3(NTHREADS), 0, s, (const float*)L, ld,
+                           dinv, k, (float*)S, lds, lower_rhs);
+    }
+    return 0;
+  }
+  const int nq = (nb + TRSM_G - 1) / TRSM_G;
+  for (int q = 0; q < nq; ++q) {
+    const dim3 g((unsigned)trsm_units(nb, q, nct));
     if (prec == PGP_PREC_FP64)
       hipLaunchKernelGGL(k_trsm_raw<double>, g, dim3(NTHREADS), 0, s, (const double*)L, ld, dinv,
-                         k, (double*)S, lds, lower_rhs);
+                         nb, q, (double*)S, lds, nct, lower_rhs);
     else
-      hipLaunchKernelGGL(k_trsm_raw<float>, g, dim3(NTHREADS), 0, s, (const float*)L, ld, dinv, k,
-                         (float*)S, lds, lower_rhs);
+      hipLaunchKernelGGL(k_trsm_raw<float>, g, dim3(NTHREADS), 0, s, (const float*)L, ld, dinv,
+                         nb, q, (float*)S, lds, nct, lower_rhs);
   }
   return 0;
 }
