@@ -94,8 +94,13 @@ typedef struct pgp_fit_options {
   int32_t maxiter;   /* 100 = jaxopt's default */
   int32_t precision; /* PGP_PREC_* */
   int32_t refine;    /* FP64 iterative-refinement steps per solve */
-  int32_t reserved;
+  int32_t flags;     /* PGP_FIT_* */
 } pgp_fit_options_t;
+
+/* pgp_batcher_launch splits a Newton step of 16 or more items into two
+ * halves on two streams, so that the latency-bound launches of one half
+ * overlap the other's work (per-kernel times then include the sharing). */
+#define PGP_FIT_SPLIT_STEPS 1
 
 /* Host-side result of a fit. */
 typedef struct pgp_fit_result {

@@ -32,6 +32,7 @@ INFO_NEGATIVE_W = 2
 INFO_MAXITER = 4
 INFO_STALLED = 8
 INFO_ESCALATED = 16
+FIT_SPLIT_STEPS = 1
 
 TIMER_SYRK = 0
 TIMER_PANEL = 1
@@ -71,7 +72,7 @@ class FitOptionsT(C.Structure):
         ("maxiter", C.c_int32),
         ("precision", C.c_int32),
         ("refine", C.c_int32),
-        ("reserved", C.c_int32),
+        ("flags", C.c_int32),
     ]
 
 

@@ -892,7 +892,7 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
 // ===========================================================================
 namespace pgp {
 
-constexpr int BATCH_ARRAYS = 6;
+constexpr int BATCH_ARRAYS = 8;
 
 struct BItem {
   pgp_batch_item_t* item;
@@ -910,10 +910,12 @@ struct BItem {
 
 struct pgp_batcher {
   hipStream_t s;
-  // finalisation and predict of converged members run on their own stream,
-  // overlapping the Newton step of the others (disjoint members)
-  hipStream_t s2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // finalisation and predict of converged members run on their own stream
+  // (s2), and a large Newton step is split in two interleaved halves, the
+  // second on s3: the three member sets are disjoint, and the latency-bound
+  // launches of one overlap the others' work
+  hipStream_t s2 = nullptr, s3 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
   pgp_fit_options_t opts;
   int refine;
   int cap;
@@ -921,7 +923,8 @@ struct pgp_batcher {
   double* gather;      // per-member step norms (device)
   double* gather_fin;  // 8 doubles per finalised member (device)
   uint32_t* ticket;   // start-order ticket of launches on s
-  uint32_t* ticket2;  // ... and on s2 (separate cache line)
+  uint32_t* ticket2;  // ... on s2, and on s3 (separate cache lines)
+  uint32_t* ticket3;
   Prob* ddesc;            // device staging of descriptors (2 * cap)
   Prob* hdesc = nullptr;  // pinned host staging of descriptors (2 * cap)
   // asynchronous launch / finish
@@ -962,9 +965,17 @@ static void write_probs(pgp_batcher* bt, const std::vector<BItem*>& v, Prob* con
 
 // Members are ordered by padded size, largest first (stable), so that the
 // members still active at any block column form a prefix of the batch.
-// Arrays 3..5 belong to the finalisation stream s2, the others to s.
+// Arrays 3..5 belong to the finalisation stream s2, 6..7 to the second
+// Newton half on s3, the others to s.
+// Newton steps of at least this many members are split over two streams
+// when the options ask for it (PGP_FIT_SPLIT_STEPS).
+constexpr int SPLIT_MIN = 16;
+
+static hipStream_t stream_of(const pgp_batcher* bt, int which) {
+  return which >= 6 ? bt->s3 : (which >= 3 ? bt->s2 : bt->s);
+}
+
 static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
-  const bool fin = which >= 3;
   std::stable_sort(v.begin(), v.end(),
                    [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
   std::vector<Prob*> ptrs;
@@ -980,14 +991,14 @@ static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
     hm.push_back(b->p.M);
     maxNp = std::max(maxNp, (int)b->p.Np);
   }
-  write_ptrs(bt->arr[which], ptrs, fin ? bt->s2 : bt->s);
+  write_ptrs(bt->arr[which], ptrs, stream_of(bt, which));
   Batch b;
   b.dprobs = bt->arr[which];
   b.count = (int)v.size();
   b.maxNp = maxNp;
   b.hnp = hnp.data();
   b.hm = hm.data();
-  b.ticket = fin ? bt->ticket2 : bt->ticket;
+  b.ticket = which >= 6 ? bt->ticket3 : (which >= 3 ? bt->ticket2 : bt->ticket);
   return b;
 }
 
@@ -1114,6 +1125,7 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
   b->ticket = (uint32_t*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
                           2 * align256((int64_t)capacity * 8 * 8));
   b->ticket2 = b->ticket + 32;
+  b->ticket3 = b->ticket + 48;
   b->ddesc = (Prob*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
                      2 * align256((int64_t)capacity * 8 * 8) + 256);
   b->hg.resize((size_t)capacity * 8);
@@ -1125,7 +1137,9 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
       hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join3, hipEventDisableTiming) != hipSuccess ||
       hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->s3, hipStreamNonBlocking) != hipSuccess ||
       hipStreamSynchronize(b->s) != hipSuccess) {
     pgp_batcher_destroy(b);
     return fail(-100, "batcher set-up (table clear, pinned buffers, events, stream) failed");
@@ -1216,35 +1230,60 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
   admit(bt);
   bt->fin_list.clear();
   bt->act_list.clear();
-  std::vector<BItem*> fac;
   for (BItem& b : bt->items) {
-    if (b.phase == 2) {
+    if (b.phase == 2)
       bt->fin_list.push_back(&b);
-    } else if (b.phase == 1) {
+    else if (b.phase == 1)
       bt->act_list.push_back(&b);
-      if (!b.gauss || b.iter == 0) fac.push_back(&b);
-    }
   }
-  const bool fork = !bt->fin_list.empty();
-  if (fork) {
-    // s2 follows everything enqueued on s so far, then runs beside the
-    // Newton step below; s joins it before the completion event
+  // a large step is split in two halves of alternating size rank, so both
+  // carry the same mix of sizes; act_list becomes half A then half B (the
+  // order of the gathered step norms)
+  // (make_batch keeps this order: it is already sorted by size)
+  std::vector<BItem*> half[2], fac[2];
+  std::stable_sort(bt->act_list.begin(), bt->act_list.end(),
+                   [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
+  if ((bt->opts.flags & PGP_FIT_SPLIT_STEPS) && (int)bt->act_list.size() >= SPLIT_MIN) {
+    for (size_t i = 0; i < bt->act_list.size(); ++i) half[i & 1].push_back(bt->act_list[i]);
+    bt->act_list = half[0];
+    bt->act_list.insert(bt->act_list.end(), half[1].begin(), half[1].end());
+  } else {
+    half[0] = bt->act_list;
+  }
+  for (int h = 0; h < 2; ++h)
+    for (BItem* b : half[h])
+      if (!b->gauss || b->iter == 0) fac[h].push_back(b);
+  const bool fin = !bt->fin_list.empty(), second = !half[1].empty();
+  if (fin || second) {
+    // s2 / s3 follow everything enqueued on s so far, then run beside the
+    // Newton step on s; s joins them before the completion event
     HIP_OK(hipEventRecord(bt->ev_fork, s));
-    HIP_OK(hipStreamWaitEvent(bt->s2, bt->ev_fork, 0));
+    if (fin) HIP_OK(hipStreamWaitEvent(bt->s2, bt->ev_fork, 0));
+    if (second) HIP_OK(hipStreamWaitEvent(bt->s3, bt->ev_fork, 0));
+  }
+  if (fin) {
     finalize_and_predict(bt, bt->fin_list, bt->gather_fin);
     HIP_OK(hipEventRecord(bt->ev_join, bt->s2));
     for (BItem* b : bt->fin_list) b->phase = 3;
   }
-  if (!bt->act_list.empty()) {
-    const Batch ba = make_batch(bt, 1, bt->act_list);
-    const Batch bf = make_batch(bt, 2, fac);
+  if (second) {
+    const Batch ba = make_batch(bt, 6, half[1]);
+    const Batch bf = make_batch(bt, 7, fac[1]);
+    newton_iter(ba, &bf, prec, bt->refine, bt->gather + half[0].size(), bt->s3);
+    HIP_OK(hipEventRecord(bt->ev_join3, bt->s3));
+  }
+  if (!half[0].empty()) {
+    const Batch ba = make_batch(bt, 1, half[0]);
+    const Batch bf = make_batch(bt, 2, fac[0]);
     newton_iter(ba, &bf, prec, bt->refine, bt->gather, s);
+  }
+  if (second) HIP_OK(hipStreamWaitEvent(s, bt->ev_join3, 0));
+  if (!bt->act_list.empty())
     HIP_OK(hipMemcpyAsync(bt->hstep, bt->gather, bt->act_list.size() * 8,
                           hipMemcpyDeviceToHost, s));
-  }
   int rc = check_launch();
   if (rc) return rc;
-  if (fork) HIP_OK(hipStreamWaitEvent(s, bt->ev_join, 0));
+  if (fin) HIP_OK(hipStreamWaitEvent(s, bt->ev_join, 0));
   HIP_OK(hipEventRecord(bt->ev, s));
   bt->launched = true;
   return 0;
@@ -1274,13 +1313,15 @@ int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
 int pgp_batcher_destroy(pgp_batcher_t* bt) {
   if (!bt) return 0;
   if (bt->launched) (void)hipEventSynchronize(bt->ev);
-  if (bt->s2) {
-    (void)hipStreamSynchronize(bt->s2);
-    (void)hipStreamDestroy(bt->s2);
-  }
+  for (hipStream_t x : {bt->s2, bt->s3})
+    if (x) {
+      (void)hipStreamSynchronize(x);
+      (void)hipStreamDestroy(x);
+    }
   if (bt->ev) (void)hipEventDestroy(bt->ev);
   if (bt->ev_fork) (void)hipEventDestroy(bt->ev_fork);
   if (bt->ev_join) (void)hipEventDestroy(bt->ev_join);
+  if (bt->ev_join3) (void)hipEventDestroy(bt->ev_join3);
   if (bt->hstep) (void)hipHostFree(bt->hstep);
   if (bt->hfin) (void)hipHostFree(bt->hfin);
   if (bt->hdesc) (void)hipHostFree(bt->hdesc);

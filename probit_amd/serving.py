@@ -41,7 +41,7 @@ class ServingEngine:
     """One process, one GPU: a batcher plus an HBM-budget admitter."""
 
     def __init__(self, precision="bf16", capacity=256, hbm_budget_bytes=None, tolerance=1e-5,
-                 maxiter=100, refine=None, device="cuda"):
+                 maxiter=100, refine=None, device="cuda", split_steps=False):
         self.lib = _lib.lib()
         self.device = torch.device(device)
         self.prec = _lib.PRECISIONS[precision]
@@ -56,6 +56,7 @@ class ServingEngine:
         opts.maxiter = maxiter
         opts.precision = self.prec
         opts.refine = -1 if refine is None else int(refine)
+        opts.flags = _lib.FIT_SPLIT_STEPS if split_steps else 0
         self._opts = opts
         tb = self.lib.pgp_batcher_table_size(self.capacity)
         self._table = _lib.workspace(tb, self.device)

@@ -182,10 +182,13 @@ def test_trsv_matches_scipy(prec, trans):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_trsm_matches_scipy(prec):
+@pytest.mark.parametrize("N,M", [(256, 192), (640, 1088)])
+def test_trsm_matches_scipy(prec, N, M):
+    """Narrow right-hand sides take the per-block-row schedule, wide ones (>= 16
+    column tiles) the super-step schedule; N = 640 ends in a partial
+    super-step."""
     torch, _lib, lib = _setup()
     rng = np.random.default_rng(8)
-    N, M = 256, 192
     L = np.linalg.cholesky(_spd(rng, N))
     S = rng.normal(size=(N, M))
     tdt = torch.float64 if prec == 0 else torch.float32

@@ -255,22 +255,29 @@ def test_serving_schedule_independent(mode):
         assert np.array_equal(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy()), k
 
 
-def test_serving_async_launch_finish_matches_step():
+@pytest.mark.parametrize("count,capacity", [(4, 3), (40, 32)])
+def test_serving_async_launch_finish_matches_step(count, capacity):
     """The asynchronous launch / finish pair gives bit-identical results to
-    the synchronous step (items are reported one step later)."""
+    the synchronous step (items are reported one step later).  With 32 in
+    flight the asynchronous Newton steps are split over two streams."""
     from probit_amd.serving import ServingEngine
+    shapes = [(150, 2, "probit", "EQ"), (300, 1, "gaussian", "Matern12"),
+              (70, 3, "probit", "Matern12"), (260, 2, "gaussian", "EQ")]
+    rng = np.random.default_rng(80)
+    while len(shapes) < count:
+        k = len(shapes)
+        shapes.append((int(rng.integers(40, 400)), int(rng.integers(1, 4)),
+                       ("probit", "gaussian")[k % 2], ("EQ", "Matern12")[k % 3 % 2]))
     probs = []
-    for i, (N, D, lik, kind) in enumerate([(150, 2, "probit", "EQ"), (300, 1, "gaussian",
-                                                                      "Matern12"),
-                                           (70, 3, "probit", "Matern12"),
-                                           (260, 2, "gaussian", "EQ")]):
+    for i, (N, D, lik, kind) in enumerate(shapes):
         X, y, theta, lp, spec = small_problem(60 + i, N, D, lik, kind)
         Xt = np.random.default_rng(70 + i).normal(size=(90, D))
         probs.append((X, y, theta, lp, lik, kind, Xt))
 
     def run(async_mode):
-        eng = ServingEngine(precision="fp32", capacity=3, hbm_budget_bytes=1 << 32)
-        out, queue = {}, list(range(4))
+        eng = ServingEngine(precision="fp32", capacity=capacity, hbm_budget_bytes=1 << 32,
+                            split_steps=True)
+        out, queue = {}, list(range(count))
         while queue or eng.in_flight:
             launched = eng.launch() if async_mode else False
             while queue:
@@ -286,8 +293,8 @@ def test_serving_async_launch_finish_matches_step():
         return out
 
     a, b = run(False), run(True)
-    assert sorted(a) == sorted(b) == [0, 1, 2, 3]
-    for t in range(4):
+    assert sorted(a) == sorted(b) == list(range(count))
+    for t in range(count):
         assert a[t].iterations == b[t].iterations
         for k in ("mean", "var", "weight", "precision"):
             assert np.array_equal(getattr(a[t], k).cpu().numpy(), getattr(b[t], k).cpu().numpy())
