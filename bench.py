@@ -264,6 +264,7 @@ def main():
 
     cpu = cpu_baseline(order[W * R:], args.cpu_seconds)
 
+    superstep = int(os.environ.get("PGP_SUPERSTEP", "8"))
     ms_per_launch = ms.value / max(1, n_l.value)
     achieved = fl.value / (ms.value * 1e-3) / 1e12 if ms.value > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
@@ -320,7 +321,8 @@ def main():
         "max_rel_mean_err_vs_fp64": float(np.max(errs)),
         "mean_newton_iterations": round(float(np.mean(iters)), 2),
         "roofline": {
-            "kernel": f"k_syrk_outer ({args.precision} rank-256 trailing update of the Cholesky)",
+            "kernel": f"k_syrk_outer ({args.precision} rank-{64 * superstep} trailing update of "
+                      "the Cholesky)",
             "bound": "mfma",
             "achieved": round(achieved, 3),
             "peak": peak,

@@ -706,14 +706,15 @@ static int fuse_max() {
   return v;
 }
 
-// Super-step width in 64-column blocks (PGP_SUPERSTEP, default 4): the
+// Super-step width in 64-column blocks (PGP_SUPERSTEP, default 8): the
 // trailing matrix right of a super-step is updated once per super-step with
 // rank 64 * width, so it is read and written width times less often than
-// with rank-64 updates.
+// with rank-64 updates, and each output tile of the update amortises its
+// start-up over width chunks of products.
 static int superstep() {
   static const int v = [] {
     const char* e = getenv("PGP_SUPERSTEP");
-    const int w = e ? atoi(e) : 4;
+    const int w = e ? atoi(e) : 8;
     return w < 1 ? 1 : (w > 16 ? 16 : w);
   }();
   return v;
