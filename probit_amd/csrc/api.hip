@@ -72,7 +72,7 @@ constexpr int64_t OFF_TICKET = 2504;    // uint32 workgroup-order counter (singl
 constexpr int64_t HEADER = 4096;
 
 struct WsLayout {
-  int64_t Np, K, L, logdiag, diagval, dinv, vec, total;
+  int64_t Np, K, L, logdiag, diagval, dinv, vec, symv, total;
 };
 
 static WsLayout ws_layout(int64_t N, int prec) {
@@ -91,6 +91,8 @@ static WsLayout ws_layout(int64_t N, int prec) {
   off += align256(w.Np * TILE * 8);
   w.vec = off;
   off += align256((int64_t)NVEC * w.Np * 8);
+  w.symv = off;
+  off += align256(w.Np * (w.Np / TILE) * 8);
   w.total = off;
   return w;
 }
@@ -272,6 +274,7 @@ static Prob make_prob(const pgp_problem_t* pr, const pgp_kernel_t* k, const pgp_
   p.diagval = (double*)(ws + L.diagval);
   p.dinv = (double*)(ws + L.dinv);
   p.vec = (double*)(ws + L.vec);
+  p.symv = (double*)(ws + L.symv);
   p.red = (double*)(ws + OFF_RED);
   p.info = (int32_t*)(ws + OFF_INFO);
   p.progress = (uint64_t*)(ws + OFF_PROGRESS);

@@ -84,6 +84,8 @@ struct Prob {
   double* C;  // Np x Np FP64 (C = W^1/2 B^-1 W^1/2, or (sigma^2 I + K)^-1)
   // single-launch triangular solves: (epoch << 32) | blocks published
   uint64_t* progress;
+  // symmetric products: per-tile partial sums, (Np/64) x (Np/64) x 64
+  double* symv;
 };
 
 __host__ __device__ inline double* vslot(const Prob& p, int v) {

@@ -1,8 +1,9 @@
 // FP64 vector work of the Newton / fixed-point loops and the predict epilogue.
 //
-//  * symmetric GEMV y_i = sum_j K_ji x_j: one wave per output reads one
-//    contiguous column of K (coalesced, deterministic wave reduction) and
-//    applies a fused epilogue (f = K a, S .* K r, refinement residuals).
+//  * symmetric GEMV y = K x: each lower 64x64 tile of K is read once (row
+//    and column sums into per-tile slots, then a fixed-order reduction with
+//    a fused epilogue: f = K a, S .* K r, refinement residuals), so HBM
+//    reads half of K per product.
 //    Replaces `K @ weight` (probit/implicit/Laplace.py:8,22,
 //    probit/approximators.py:273,338) and the matvecs inside the reference's
 //    dense Jacobian (probit/implicit/solvers.py:24).
@@ -24,28 +25,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // mode 1: out = S .* Kx
 // mode 2: out = R - DL - S^2 .* Kx     (Laplace residual of (I + W K) d = r, x = DL)
 // mode 3: out = X1 - sigma^2 x - Kx    (VB residual of (sigma^2 I + K) w = rhs, rhs in X1)
-// mode 4: out = x + Kx... unused
-__global__ __launch_bounds__(NTHREADS) void k_gemv_sym(Prob* const* probs, Sched sc, int xs,
-                                                       int os, int mode) {
-  int bx;
-  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
-  const int i = bx * 4 + (threadIdx.x >> 6);
-  if (i >= P.Np) return;
-  const int lane = threadIdx.x & 63;
-  const double* col = (mode >= 10 ? P.C : P.K) + (size_t)i * P.Np;
-  mode %= 10;
-  const double* x = vslot(P, xs);
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int j = lane;
-  for (; j + 192 < P.Np; j += 256) {
-    a0 += col[j] * x[j];
-    a1 += col[j + 64] * x[j + 64];
-    a2 += col[j + 128] * x[j + 128];
-    a3 += col[j + 192] * x[j + 192];
-  }
-  for (; j < P.Np; j += 64) a0 += col[j] * x[j];
-  const double sum = wave_sum((a0 + a1) + (a2 + a3));
-  if (lane != 0) return;
+// (modes >= 10: the same with the matrix C in place of K)
+__device__ __forceinline__ void gemv_epilogue(const Prob& P, int i, const double* x, int os,
+                                              int mode, double sum) {
   double v;
   switch (mode) {
     case 0:
@@ -68,11 +50,124 @@ __global__ __launch_bounds__(NTHREADS) void k_gemv_sym(Prob* const* probs, Sched
   vslot(P, os)[i] = v;
 }
 
+// Full-matrix product, one wave per output (used for the matrix C).
+__global__ __launch_bounds__(NTHREADS) void k_gemv_sym(Prob* const* probs, Sched sc, int xs,
+                                                       int os, int mode) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int i = bx * 4 + (threadIdx.x >> 6);
+  if (i >= P.Np) return;
+  const int lane = threadIdx.x & 63;
+  const double* col = (mode >= 10 ? P.C : P.K) + (size_t)i * P.Np;
+  mode %= 10;
+  const double* x = vslot(P, xs);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int j = lane;
+  for (; j + 192 < P.Np; j += 256) {
+    a0 += col[j] * x[j];
+    a1 += col[j + 64] * x[j + 64];
+    a2 += col[j + 128] * x[j + 128];
+    a3 += col[j + 192] * x[j + 192];
+  }
+  for (; j < P.Np; j += 64) a0 += col[j] * x[j];
+  const double sum = wave_sum((a0 + a1) + (a2 + a3));
+  if (lane == 0) gemv_epilogue(P, i, x, os, mode, sum);
+}
+
+// Symmetric product y = K x reading each lower 64x64 tile of K once: the
+// tile (I, J) gives its row sums K_IJ x_J (towards y_I) and, off the
+// diagonal, its column sums K_IJ^T x_I (towards y_J), each written to its
+// own slot part[I][J] / part[J][I] (64 doubles).  k_symv_reduce then adds
+// the nb slots of every row block in a fixed order, so the result does not
+// depend on scheduling, and applies the epilogue.
+__device__ __forceinline__ void lower_tile_v(int t, int& ti, int& tj) {
+  int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  while (i * (i + 1) / 2 > t) --i;
+  ti = i;
+  tj = t - i * (i + 1) / 2;
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_symv_tiles(Prob* const* probs, Sched sc, int xs) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int nb = P.Np / TILE;
+  if (bx >= nb * (nb + 1) / 2) return;
+  int ti, tj;
+  lower_tile_v(bx, ti, tj);
+  __shared__ double t[TILE][TILE + 1];
+  __shared__ double xi[TILE], xj[TILE];
+  __shared__ double rpart[4][TILE], cpart[4][TILE];
+  const size_t ld = (size_t)P.Np;
+  const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const double* x = vslot(P, xs);
+  const double* Kt = P.K + (size_t)ti * TILE + (size_t)tj * TILE * ld;
+#pragma unroll 4
+  for (int q = 0; q < 16; ++q) {
+    const int c = w + 4 * q;
+    t[r][c] = Kt[(size_t)r + (size_t)c * ld];  // lane = row: coalesced down a column
+  }
+  if (w == 0) xi[r] = x[ti * TILE + r];
+  if (w == 1) xj[r] = x[tj * TILE + r];
+  __syncthreads();
+  double rs = 0.0, cs = 0.0;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int k = 16 * w + u;
+    rs += t[r][k] * xj[k];  // row r, columns 16w..
+    cs += t[k][r] * xi[k];  // column r, rows 16w..
+  }
+  rpart[w][r] = rs;
+  cpart[w][r] = cs;
+  __syncthreads();
+  double* part = P.symv;
+  if (w == 0)
+    part[((size_t)ti * nb + tj) * TILE + r] = (rpart[0][r] + rpart[1][r]) + (rpart[2][r] + rpart[3][r]);
+  else if (w == 1 && ti != tj)
+    part[((size_t)tj * nb + ti) * TILE + r] = (cpart[0][r] + cpart[1][r]) + (cpart[2][r] + cpart[3][r]);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_symv_reduce(Prob* const* probs, Sched sc, int xs,
+                                                          int os, int mode) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int i = bx * NTHREADS + threadIdx.x;
+  if (i >= P.Np) return;
+  const int nb = P.Np / TILE;
+  const double* part = P.symv + (size_t)(i / TILE) * nb * TILE + (i % TILE);
+  double a0 = 0.0, a1 = 0.0;
+  int J = 0;
+  for (; J + 1 < nb; J += 2) {
+    a0 += part[(size_t)J * TILE];
+    a1 += part[(size_t)(J + 1) * TILE];
+  }
+  if (J < nb) a0 += part[(size_t)J * TILE];
+  gemv_epilogue(P, i, vslot(P, xs), os, mode, a0 + a1);
+}
+
 void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s) {
+  if (mode >= 10) {
+    sched_launch(
+        b, [&](int m) { return (b.np(m) + 3) / 4; },
+        [&](Prob* const* dp, const Sched& sc, int n) {
+          hipLaunchKernelGGL(k_gemv_sym, dim3(n), dim3(NTHREADS), 0, s, dp, sc, x_slot, out_slot,
+                             mode);
+        });
+    return;
+  }
   sched_launch(
-      b, [&](int m) { return (b.np(m) + 3) / 4; },
+      b,
+      [&](int m) {
+        const int t = b.np(m) / TILE;
+        return t * (t + 1) / 2;
+      },
       [&](Prob* const* dp, const Sched& sc, int n) {
-        hipLaunchKernelGGL(k_gemv_sym, dim3(n), dim3(NTHREADS), 0, s, dp, sc, x_slot, out_slot,
+        hipLaunchKernelGGL(k_symv_tiles, dim3(n), dim3(NTHREADS), 0, s, dp, sc, x_slot);
+      });
+  sched_launch(
+      b, [&](int m) { return (b.np(m) + NTHREADS - 1) / NTHREADS; },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        hipLaunchKernelGGL(k_symv_reduce, dim3(n), dim3(NTHREADS), 0, s, dp, sc, x_slot, out_slot,
                            mode);
       });
 }
