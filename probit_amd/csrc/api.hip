@@ -320,13 +320,15 @@ static void solve_B(const Batch& b, int prec, hipStream_t s) {
 }
 
 // One Newton iteration z <- z + (I + W K)^-1 (grad - z) for every member of
-// `all`; members of `fac` re-form and re-factor B first.  ||delta||^2 goes to
-// gather[b].
-static void newton_iter(const Batch& all, const Batch* fac, int prec, int refine, double* gather,
-                        hipStream_t s) {
+// `all`; members of `fac` re-form and re-factor B first (those of `fac32`
+// with FP32 updates: BF16 members escalated after a loss of definiteness;
+// both store the factor in FP32).  ||delta||^2 goes to gather[b].
+static void newton_iter(const Batch& all, const Batch* fac, const Batch* fac32, int prec,
+                        int refine, double* gather, hipStream_t s) {
   launch_gemv_sym(all, V_Z, V_F, 0, s);
   launch_lik_newton(all, s);
   if (fac && fac->count > 0) factor(*fac, prec, 0, s);
+  if (fac32 && fac32->count > 0) factor(*fac32, PGP_PREC_FP32, 0, s);
   launch_gemv_sym(all, V_R, V_T, 1, s);
   solve_B(all, prec, s);
   launch_vec_op(all, OP_STEP, s);
@@ -341,11 +343,12 @@ static void newton_iter(const Batch& all, const Batch* fac, int prec, int refine
 }
 
 // Final state at the returned weight: f, W, sum log p, and B factored at W.
-static void laplace_finalize(const Batch& all, const Batch* fac, int prec, double* gather,
-                             hipStream_t s) {
+static void laplace_finalize(const Batch& all, const Batch* fac, const Batch* fac32, int prec,
+                             double* gather, hipStream_t s) {
   launch_gemv_sym(all, V_Z, V_F, 0, s);
   launch_lik_final(all, 0, s);
   if (fac && fac->count > 0) factor(*fac, prec, 0, s);
+  if (fac32 && fac32->count > 0) factor(*fac32, PGP_PREC_FP32, 0, s);
   launch_reduce(all, RED_FINAL, gather, s);
   launch_vec_op(all, OP_OUT, s);
 }
@@ -366,6 +369,12 @@ static void fill_result_final(pgp_fit_result_t* r, const double* g4, int info) {
 // no refinement costs ~10% more iterations but half the solves per
 // iteration.  BF16 trailing updates need the refinement.
 static int default_refine(int prec) { return prec == PGP_PREC_BF16 ? 2 : 0; }
+
+// Inexact-Newton stall test of the BF16 mode: from the third iteration on, a
+// step norm above half the previous one.
+static bool stalled(int iteration, double step, double prev) {
+  return iteration >= 3 && step > 0.5 * prev;
+}
 
 // ---------------------------------------------------------------------------
 // opt-in launch timer (HIP events on the launching stream)
@@ -574,9 +583,11 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   double step = INFINITY;
   const bool gauss = lik->kind == PGP_LIK_GAUSSIAN;
   int cur = prec;  // BF16 escalates to FP32 (same storage) if its factor fails
+  bool force_factor = false;
   while (step > opts->tolerance && it < opts->maxiter) {
-    const bool refactor = !gauss || it == 0;
-    newton_iter(b, refactor ? &b : nullptr, cur, refine, gather, s);
+    const bool refactor = !gauss || it == 0 || force_factor;
+    force_factor = false;
+    newton_iter(b, refactor ? &b : nullptr, nullptr, cur, refine, gather, s);
     rc = check_launch();
     if (rc) return rc;
     double h2;
@@ -584,21 +595,29 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
     HIP_OK(hipMemcpyAsync(&h2, gather, sizeof h2, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&inf0, ws + OFF_INFO, sizeof inf0, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    if ((inf0 & PGP_INFO_NOT_PD) && cur == PGP_PREC_BF16) {
+    const double st = sqrt(h2);
+    if (cur == PGP_PREC_BF16 && ((inf0 & PGP_INFO_NOT_PD) || !(st == st))) {
       // the step was not taken (OP_ZUPD); redo the iteration with FP32 updates
       cur = PGP_PREC_FP32;
       result->status |= PGP_INFO_ESCALATED;
+      force_factor = true;
       HIP_OK(hipMemsetAsync(ws + OFF_INFO, 0, sizeof(int32_t), s));
       continue;
     }
-    step = sqrt(h2);
+    if (cur == PGP_PREC_BF16 && stalled(it + 1, st, step)) {
+      // the inexact Newton stalls: go on with FP32 updates
+      cur = PGP_PREC_FP32;
+      result->status |= PGP_INFO_ESCALATED;
+      force_factor = true;
+    }
+    step = st;
     ++it;
     if (!(step == step) || (inf0 & PGP_INFO_NOT_PD)) break;  // stop, report through status
   }
   result->iterations = it;
   result->step_norm = step;
   if (step > opts->tolerance) result->status |= PGP_INFO_MAXITER;
-  laplace_finalize(b, gauss ? nullptr : &b, cur, gather, s);
+  laplace_finalize(b, gauss ? nullptr : &b, nullptr, cur, gather, s);
   rc = check_launch();
   if (rc) return rc;
   double g4[4];
@@ -895,7 +914,11 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
 // ===========================================================================
 namespace pgp {
 
-constexpr int BATCH_ARRAYS = 8;
+// Device pointer arrays of the batcher: 0 admitted / flag clearing, 1-2
+// Newton half A (all, factored), 3-5 finalisation (all, factored,
+// predicted), 6-7 Newton half B, 8-10 the FP32-escalated factored members of
+// half A, half B and the finalisation.
+constexpr int BATCH_ARRAYS = 11;
 
 struct BItem {
   pgp_batch_item_t* item;
@@ -907,6 +930,10 @@ struct BItem {
   double step;
   bool gauss;
   bool done;
+  bool esc = false;    // BF16 batcher: factored with FP32 updates from now on
+  bool clear = false;  // its NOT_PD flag is to be cleared before the next step
+  bool refac = false;  // re-factor at the next step (escalated Gaussian member)
+  double prev = INFINITY;  // step norm of the previous iteration
 };
 
 }  // namespace pgp
@@ -974,8 +1001,13 @@ static void write_probs(pgp_batcher* bt, const std::vector<BItem*>& v, Prob* con
 // when the options ask for it (PGP_FIT_SPLIT_STEPS).
 constexpr int SPLIT_MIN = 16;
 
+static int lane_of(int which) {  // 0: s, 1: s2, 2: s3
+  static const int lane[BATCH_ARRAYS] = {0, 0, 0, 1, 1, 1, 2, 2, 0, 2, 1};
+  return lane[which];
+}
 static hipStream_t stream_of(const pgp_batcher* bt, int which) {
-  return which >= 6 ? bt->s3 : (which >= 3 ? bt->s2 : bt->s);
+  const int l = lane_of(which);
+  return l == 2 ? bt->s3 : (l == 1 ? bt->s2 : bt->s);
 }
 
 static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
@@ -1001,7 +1033,8 @@ static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
   b.maxNp = maxNp;
   b.hnp = hnp.data();
   b.hm = hm.data();
-  b.ticket = which >= 6 ? bt->ticket3 : (which >= 3 ? bt->ticket2 : bt->ticket);
+  const int l = lane_of(which);
+  b.ticket = l == 2 ? bt->ticket3 : (l == 1 ? bt->ticket2 : bt->ticket);
   return b;
 }
 
@@ -1033,12 +1066,13 @@ static void admit(pgp_batcher* bt) {
 static void finalize_and_predict(pgp_batcher* bt, std::vector<BItem*>& conv, double* gather) {
   hipStream_t s = bt->s2;
   const int prec = bt->opts.precision;
-  std::vector<BItem*> conv_fac, pred;
+  std::vector<BItem*> conv_fac, conv_fac32, pred;
   for (BItem* b : conv)
-    if (!b->gauss) conv_fac.push_back(b);
+    if (!b->gauss) (b->esc ? conv_fac32 : conv_fac).push_back(b);
   const Batch bc = make_batch(bt, 3, conv);
   const Batch bcf = make_batch(bt, 4, conv_fac);
-  laplace_finalize(bc, &bcf, prec, gather, s);
+  const Batch bcf32 = make_batch(bt, 10, conv_fac32);
+  laplace_finalize(bc, &bcf, &bcf32, prec, gather, s);
   int maxM = 0;
   for (BItem* b : conv) {
     pgp_batch_item_t* it = b->item;
@@ -1092,6 +1126,44 @@ static void drop_done(pgp_batcher* bt) {
   for (BItem& b : bt->items)
     if (!b.done) keep.push_back(b);
   bt->items.swap(keep);
+}
+
+// BF16 batcher, as the single-problem fit: a member whose factor lost
+// definiteness (NaN step; its weight was left unchanged) redoes the
+// iteration with FP32 updates; one whose inexact Newton stalls (from the
+// third iteration, a step norm above half the previous one) continues with
+// FP32 updates from its current weight.  Returns true if the iteration is
+// to be redone (not counted).
+static bool escalate(const pgp_batcher* bt, BItem* b) {
+  if (bt->opts.precision != PGP_PREC_BF16 || b->esc) return false;
+  const bool failed = !(b->step == b->step);
+  if (!failed && !stalled(b->iter + 1, b->step, b->prev)) return false;
+  b->esc = true;
+  b->refac = true;
+  b->clear = failed;
+  b->item->result.status |= PGP_INFO_ESCALATED;
+  if (failed) b->step = INFINITY;
+  return failed;
+}
+
+__global__ void k_clear_not_pd(Prob* const* arr, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) arr[i]->info[0] = 0;
+}
+
+// Clear the flags of members escalated in the previous step (array 0, after
+// the admission's launches on the same stream).
+static void clear_escalated(pgp_batcher* bt) {
+  std::vector<BItem*> clr;
+  for (BItem& b : bt->items)
+    if (b.clear) {
+      clr.push_back(&b);
+      b.clear = false;
+    }
+  if (clr.empty()) return;
+  const Batch bz = make_batch(bt, 0, clr);
+  hipLaunchKernelGGL(k_clear_not_pd, dim3((bz.count + 255) / 256), dim3(256), 0, bt->s, bz.dprobs,
+                     bz.count);
 }
 
 static bool converged(const pgp_batcher* bt, const BItem* b) {
@@ -1194,15 +1266,18 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   hipStream_t s = bt->s;
   const int prec = bt->opts.precision;
   admit(bt);
-  std::vector<BItem*> active, fac;
+  clear_escalated(bt);
+  std::vector<BItem*> active, fac, fac32;
   for (BItem& b : bt->items) {
     active.push_back(&b);
-    if (!b.gauss || b.iter == 0) fac.push_back(&b);
+    if (!b.gauss || b.iter == 0 || b.refac) (b.esc ? fac32 : fac).push_back(&b);
+    b.refac = false;
   }
   if (active.empty()) return 0;
   const Batch ba = make_batch(bt, 1, active);
   const Batch bf = make_batch(bt, 2, fac);
-  newton_iter(ba, &bf, prec, bt->refine, bt->gather, s);
+  const Batch bf32 = make_batch(bt, 8, fac32);
+  newton_iter(ba, &bf, &bf32, prec, bt->refine, bt->gather, s);
   int rc = check_launch();
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(bt->hstep, bt->gather, active.size() * 8, hipMemcpyDeviceToHost, s));
@@ -1211,7 +1286,9 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   for (size_t i = 0; i < active.size(); ++i) {
     BItem* b = active[i];
     b->step = sqrt(bt->hstep[i]);
+    if (escalate(bt, b)) continue;
     b->iter++;
+    b->prev = b->step;
     if (converged(bt, b)) conv.push_back(b);
   }
   if (conv.empty()) return 0;
@@ -1231,6 +1308,7 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
   hipStream_t s = bt->s;
   const int prec = bt->opts.precision;
   admit(bt);
+  clear_escalated(bt);
   bt->fin_list.clear();
   bt->act_list.clear();
   for (BItem& b : bt->items) {
@@ -1243,7 +1321,7 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
   // carry the same mix of sizes; act_list becomes half A then half B (the
   // order of the gathered step norms)
   // (make_batch keeps this order: it is already sorted by size)
-  std::vector<BItem*> half[2], fac[2];
+  std::vector<BItem*> half[2], fac[2], fac32[2];
   std::stable_sort(bt->act_list.begin(), bt->act_list.end(),
                    [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
   if ((bt->opts.flags & PGP_FIT_SPLIT_STEPS) && (int)bt->act_list.size() >= SPLIT_MIN) {
@@ -1255,7 +1333,8 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
   }
   for (int h = 0; h < 2; ++h)
     for (BItem* b : half[h])
-      if (!b->gauss || b->iter == 0) fac[h].push_back(b);
+      if (!b->gauss || b->iter == 0 || b->refac) (b->esc ? fac32[h] : fac[h]).push_back(b);
+  for (BItem* b : bt->act_list) b->refac = false;
   const bool fin = !bt->fin_list.empty(), second = !half[1].empty();
   if (fin || second) {
     // s2 / s3 follow everything enqueued on s so far, then run beside the
@@ -1272,13 +1351,15 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
   if (second) {
     const Batch ba = make_batch(bt, 6, half[1]);
     const Batch bf = make_batch(bt, 7, fac[1]);
-    newton_iter(ba, &bf, prec, bt->refine, bt->gather + half[0].size(), bt->s3);
+    const Batch bf32 = make_batch(bt, 9, fac32[1]);
+    newton_iter(ba, &bf, &bf32, prec, bt->refine, bt->gather + half[0].size(), bt->s3);
     HIP_OK(hipEventRecord(bt->ev_join3, bt->s3));
   }
   if (!half[0].empty()) {
     const Batch ba = make_batch(bt, 1, half[0]);
     const Batch bf = make_batch(bt, 2, fac[0]);
-    newton_iter(ba, &bf, prec, bt->refine, bt->gather, s);
+    const Batch bf32 = make_batch(bt, 8, fac32[0]);
+    newton_iter(ba, &bf, &bf32, prec, bt->refine, bt->gather, s);
   }
   if (second) HIP_OK(hipStreamWaitEvent(s, bt->ev_join3, 0));
   if (!bt->act_list.empty())
@@ -1304,7 +1385,9 @@ int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
   for (size_t i = 0; i < bt->act_list.size(); ++i) {
     BItem* b = bt->act_list[i];
     b->step = sqrt(bt->hstep[i]);
+    if (escalate(bt, b)) continue;
     b->iter++;
+    b->prev = b->step;
     if (converged(bt, b)) b->phase = 2;
   }
   bt->fin_list.clear();

@@ -255,6 +255,46 @@ def test_serving_schedule_independent(mode):
         assert np.array_equal(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy()), k
 
 
+def test_serving_bf16_escalates_like_single_fit():
+    """A BF16 request whose factor loses definiteness (ill-conditioned B:
+    smooth prior, large signal variance, small probit noise) is redone with
+    FP32 updates, flagged PGP_INFO_ESCALATED, and matches the single-problem
+    fit, which escalates the same way, bit for bit; well-conditioned
+    requests beside it are unaffected."""
+    import probit_amd as P
+    from probit_amd.serving import ServingEngine
+    bad = small_problem(95, 640, 1, "probit", "EQ", ls=2.0, var=10.0, noise=0.2)
+    good = [small_problem(96 + i, 200, 2, "probit", "Matern12") for i in range(2)]
+    Xt = np.random.default_rng(97).normal(size=(50, 1))
+    eng = ServingEngine(precision="bf16", capacity=4, hbm_budget_bytes=1 << 32)
+    X, y, theta, lp, spec = bad
+    assert eng.submit(0, X, y, "EQ", theta[0], theta[1], "probit", lp[0], lp[1], Xt)
+    for i, (Xg, yg, th, lpg, _) in enumerate(good):
+        Xtg = np.random.default_rng(98 + i).normal(size=(50, 2))
+        assert eng.submit(1 + i, Xg, yg, "Matern12", th[0], th[1], "probit", lpg[0], lpg[1], Xtg)
+    out = {}
+    while eng.in_flight:
+        for c in eng.step():
+            out[c.tag] = c
+    eng.close()
+    gp = _gp("LaplaceGP", X, y, "EQ", "probit", precision_mode="bf16")
+    w, p = gp.approximate_posterior((theta, lp))
+    mean, var = gp.predict(Xt, (theta, lp), w, p)
+    assert gp.last_result.status & P._lib.INFO_ESCALATED
+    c = out[0]
+    assert c.status & P._lib.INFO_ESCALATED
+    assert c.iterations == gp.last_result.iterations
+    assert np.array_equal(c.mean.cpu().numpy(), mean.cpu().numpy())
+    assert np.array_equal(c.var.cpu().numpy(), var.cpu().numpy())
+    K = O.gram(spec, X)
+    w_ref, _, _ = O.laplace_weight(K, y, O.PROBIT, lp, tol=1e-5)
+    p_ref, _ = O.laplace_precision(K, y, O.PROBIT, lp, w_ref)
+    m_ref, _ = O.predict(spec, X, Xt, w_ref, p_ref)
+    assert rel_err(c.mean.cpu().numpy(), m_ref) < 1e-5
+    for t in (1, 2):
+        assert out[t].status == 0 and np.all(np.isfinite(out[t].mean.cpu().numpy()))
+
+
 @pytest.mark.parametrize("count,capacity", [(4, 3), (40, 32)])
 def test_serving_async_launch_finish_matches_step(count, capacity):
     """The asynchronous launch / finish pair gives bit-identical results to
