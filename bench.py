@@ -7,9 +7,12 @@ or ordinal probit (J uniform on {2..5}), EQ or Matern12 with lengthscale
 log-uniform on [0.2, 2], 256 test points per request.  One posterior =
 LaplaceGP.approximate_posterior + predict for one request.
 
-A step = one window of --requests-per-step requests served to completion by
-the continuous-batching engine (closed loop: the whole window is offered at
-the start of the step; requests enter as slots and memory free up).  All
+The timed region serves --steps x --requests-per-step requests of the stream
+as one continuous stream through the continuous-batching engine (closed
+loop: every request is offered at the start; requests enter as slots and
+memory free up, so the engine stays full until the stream runs out); a step
+is --requests-per-step of them (ms_per_step = elapsed / steps).  Closed-loop
+latency is measured from a request's admission to its completion.  All
 request inputs are resident in HBM before timing starts.  Also measured,
 outside the timed region:
   * the plain per-request FP64 path (pgp_laplace_fit + pgp_predict, one
@@ -78,12 +81,13 @@ def upload(reqs, torch):
 
 
 def serve_window(engine, window, t_arrive=None, chunk=128):
-    """Serve a window to completion; returns {rid: (latency_s, Completed)}.
+    """Serve a list of requests to completion; returns {rid: (latency_s,
+    Completed)}, latency from arrival (t_arrive) or else from submission.
     Asynchronous steps: while the GPU runs a step, the host submits up to
     `chunk` more requests (admitted at the next launch)."""
     import time as _t
     pending = list(window)
-    t0 = _t.perf_counter()
+    t_sub = {}
     res = {}
     while pending or engine.in_flight:
         launched = engine.launch()
@@ -95,6 +99,7 @@ def serve_window(engine, window, t_arrive=None, chunk=128):
             p = r["p"]
             if engine.submit(r["rid"], r["X"], r["y"], p.kind, p.lengthscale, p.variance, p.lik,
                              p.noise_std, p.cutpoints, r["Xt"], p.period):
+                t_sub[r["rid"]] = _t.perf_counter()
                 n += 1
             else:
                 still.append(r)
@@ -103,7 +108,7 @@ def serve_window(engine, window, t_arrive=None, chunk=128):
             continue
         for c in engine.finish():
             t_done = _t.perf_counter()
-            start = t0 if t_arrive is None else t_arrive[c.tag]
+            start = t_sub[c.tag] if t_arrive is None else t_arrive[c.tag]
             res[c.tag] = (t_done - start, c)
     return res
 
@@ -228,8 +233,7 @@ def main():
     results = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(W, W + K):
-        results.update(serve_window(eng, dev_reqs[s * R:(s + 1) * R]))
+    results.update(serve_window(eng, dev_reqs[W * R:(W + K) * R]))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     lib.pgp_timer_enable(0)

@@ -459,6 +459,30 @@ __global__ __launch_bounds__(NTHREADS) void k_col_update(Prob* const* probs, Sch
                                  FUSE && i == j, o);
 }
 
+// Tile t of a Tk x Tk lower triangle in blocked order: super-tiles of 8 x 8
+// tiles, super-rows in order, the super-tiles of a super-row left to right
+// (the diagonal one as a lower triangle), row-major inside.  Consecutive
+// tiles (those one XCD runs together) then share a few 64-row panels, which
+// stay in that XCD's L2.  Super-rows before I hold 32 I^2 + 4 I tiles.
+__device__ __forceinline__ void blocked_tile(int t, int Tk, int& ti, int& tj) {
+  int I = (int)((sqrt(16.0 + 128.0 * (double)t) - 4.0) / 64.0);
+  while (32 * (I + 1) * (I + 1) + 4 * (I + 1) <= t) ++I;
+  while (32 * I * I + 4 * I > t) --I;
+  const int u = t - (32 * I * I + 4 * I);
+  const int rI = min(8, Tk - 8 * I);
+  const int off = rI * 8 * I;
+  if (u < off) {
+    const int J = u / (rI * 8), v = u - J * rI * 8;
+    ti = 8 * I + (v >> 3);
+    tj = 8 * J + (v & 7);
+  } else {
+    int a, c;
+    lower_tile_c(u - off, a, c);
+    ti = 8 * I + a;
+    tj = 8 * I + c;
+  }
+}
+
 // Outer update closing a super-step [s0, s_end): every lower tile (i, j),
 // s_end <= j <= i, gets A_ij -= sum_{c in [s0, s_end)} L_ic L_jc^T.
 template <typename T, int BF16, int FUSE>
@@ -471,7 +495,7 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
   const int Tk = nb - s_end;
   if (Tk <= 0 || bx >= Tk * (Tk + 1) / 2) return;
   int ti, tj;
-  lower_tile_c(bx, ti, tj);
+  blocked_tile(bx, Tk, ti, tj);
   const int c1 = s_end * TILE;
   const FactorOut o{P.L, P.Np, P.dinv + (size_t)s_end * TILE * TILE,
                     P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr,
