@@ -54,9 +54,6 @@ def parse():
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-sized device buffer 3 times and exit (the known-size "
                         "reference of the HBM traffic counters)")
-    p.add_argument("--split-steps", action="store_true",
-                   help="split large Newton steps over two streams (per-kernel times then "
-                        "include the sharing)")
     p.add_argument("--refine", type=int, default=-1,
                    help="FP64 refinement steps per Newton solve (-1: library default)")
     p.add_argument("--plain-sample", type=int, default=128)
@@ -222,8 +219,7 @@ def main():
     torch.cuda.synchronize()
 
     eng = ServingEngine(precision=args.precision, capacity=args.capacity,
-                        refine=None if args.refine < 0 else args.refine,
-                        split_steps=args.split_steps)
+                        refine=None if args.refine < 0 else args.refine)
     # warmup windows
     for s in range(W):
         serve_window(eng, dev_reqs[s * R:(s + 1) * R])
@@ -302,7 +298,7 @@ def main():
                         "EQ / Matern12, 256 test points; approximate_posterior + predict",
             "requests_per_step": R,
             "capacity": args.capacity,
-            "split_steps": bool(args.split_steps),
+            "lane_np": int(os.environ.get("PGP_LANE_NP", "1024")),
             "precision": args.precision,
             "tolerance": 1e-5,
         },

@@ -94,13 +94,8 @@ typedef struct pgp_fit_options {
   int32_t maxiter;   /* 100 = jaxopt's default */
   int32_t precision; /* PGP_PREC_* */
   int32_t refine;    /* FP64 iterative-refinement steps per solve */
-  int32_t flags;     /* PGP_FIT_* */
+  int32_t flags;     /* reserved, 0 */
 } pgp_fit_options_t;
-
-/* pgp_batcher_launch splits a Newton step of 16 or more items into two
- * halves on two streams, so that the latency-bound launches of one half
- * overlap the other's work (per-kernel times then include the sharing). */
-#define PGP_FIT_SPLIT_STEPS 1
 
 /* Host-side result of a fit. */
 typedef struct pgp_fit_result {
@@ -317,16 +312,22 @@ int pgp_batcher_submit(pgp_batcher_t* batcher, pgp_batch_item_t* item,
 int pgp_batcher_step(pgp_batcher_t* batcher, int64_t* done_tags,
                      int32_t max_done, int32_t* n_done);
 /* Asynchronous form of pgp_batcher_step, so that the host can prepare and
- * submit requests while the GPU works: launch() admits queued items and
- * enqueues one step (the finalisation and predict of the items that
- * converged in the previous step, one Newton iteration for all others) and
- * returns at once; finish() waits for that step and returns the tags of the
- * items it completed.  An item is reported one step after it converges;
- * its results are the same as with pgp_batcher_step.  Do not mix the two
- * forms while a launch is pending. */
+ * submit requests while the GPU works.  Items are split by size into two
+ * lanes (padded N above 1024, or PGP_LANE_NP in the environment, takes the
+ * second), each with its own streams and step cadence, so that the small
+ * items run several steps while the large ones run one.  launch() admits
+ * queued items and, on every lane without a step in flight, enqueues one
+ * step (the finalisation and predict of the lane's items that converged in
+ * its previous step, one Newton iteration for its other items), returning
+ * at once; finish() waits until at least one lane's step is done and returns
+ * the tags of the items it completed; launched() counts the lanes with a
+ * step in flight.  An item is reported one step after it converges; its
+ * results are the same as with pgp_batcher_step.  Do not call
+ * pgp_batcher_step while a lane has a step in flight. */
 int pgp_batcher_launch(pgp_batcher_t* batcher);
 int pgp_batcher_finish(pgp_batcher_t* batcher, int64_t* done_tags,
                        int32_t max_done, int32_t* n_done);
+int32_t pgp_batcher_launched(pgp_batcher_t* batcher);
 int32_t pgp_batcher_in_flight(pgp_batcher_t* batcher);
 int pgp_batcher_destroy(pgp_batcher_t* batcher);
 

@@ -32,7 +32,6 @@ INFO_NEGATIVE_W = 2
 INFO_MAXITER = 4
 INFO_STALLED = 8
 INFO_ESCALATED = 16
-FIT_SPLIT_STEPS = 1
 
 TIMER_SYRK = 0
 TIMER_PANEL = 1
@@ -159,6 +158,7 @@ _SIGNATURES = {
     "pgp_batcher_launch": (C.c_int, [_P]),
     "pgp_batcher_finish": (C.c_int, [_P, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int32)]),
     "pgp_batcher_in_flight": (_I32, [_P]),
+    "pgp_batcher_launched": (_I32, [_P]),
     "pgp_batcher_destroy": (C.c_int, [_P]),
 }
 

@@ -15,7 +15,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <list>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pgp_internal.hpp"
@@ -904,21 +907,28 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
 // ===========================================================================
 // Continuous batching: many fits share every launch of a Newton iteration.
 //
-// A request joins at the next step() after submit(); each step runs one
+// A request joins at the next step after submit(); each step runs one
 // Newton iteration for every request in flight (one launch per blocked
 // Cholesky / solve step for all of them together), then finalises and
 // predicts the requests that converged, which leave the batch.  Every
 // member's arithmetic is independent of the other members (per-member
 // tiles, fixed-order per-member reductions), so a request's result is the
 // same alone and in any mix.
+//
+// Asynchronous form: the members are split by size into two lanes, each with
+// its own streams and its own step cadence.  A step's duration is set by
+// its largest members' factorisations, so the lane of small members runs
+// several steps while the lane of large members runs one, and fills the GPU
+// during the large members' latency-bound column steps.
 // ===========================================================================
 namespace pgp {
 
-// Device pointer arrays of the batcher: 0 admitted / flag clearing, 1-2
-// Newton half A (all, factored), 3-5 finalisation (all, factored,
-// predicted), 6-7 Newton half B, 8-10 the FP32-escalated factored members of
-// half A, half B and the finalisation.
-constexpr int BATCH_ARRAYS = 11;
+// Device pointer arrays of a lane: 0 admitted / flag clearing, 1 Newton
+// members, 2 their factored ones, 3 finalised members, 4 their factored
+// ones, 5 predicted ones, 6 and 7 the FP32-escalated factored members of 2
+// and 4.  Arrays 3, 4, 5 and 7 belong to the lane's finalisation stream.
+constexpr int BATCH_ARRAYS = 8;
+constexpr int NLANES = 2;
 
 struct BItem {
   pgp_batch_item_t* item;
@@ -927,6 +937,7 @@ struct BItem {
   Prob p;
   int phase;  // 0 = new, 1 = Newton, 2 = converged (finalise next), 3 = finalising
   int iter;
+  int lane;   // asynchronous form: 0 small members, 1 large ones
   double step;
   bool gauss;
   bool done;
@@ -936,87 +947,71 @@ struct BItem {
   double prev = INFINITY;  // step norm of the previous iteration
 };
 
+struct Lane {
+  hipStream_t s = nullptr;   // Newton steps (lane 0: the caller's stream)
+  hipStream_t s2 = nullptr;  // finalisation and predict, beside the Newton step
+  bool own_s = false;
+  hipEvent_t ev = nullptr, ev_fork = nullptr, ev_join = nullptr;
+  Prob** arr[BATCH_ARRAYS];
+  std::vector<int> hnp[BATCH_ARRAYS], hm[BATCH_ARRAYS];
+  double* gather;      // per-member step norms (device)
+  double* gather_fin;  // 8 doubles per finalised member (device)
+  uint32_t* ticket;    // start-order tickets of launches on s and on s2
+  uint32_t* ticket2;
+  Prob* ddesc;             // device staging of descriptors (2 * cap)
+  Prob* hdesc = nullptr;   // pinned host staging of descriptors (2 * cap)
+  double* hstep = nullptr;  // pinned copies of gather / gather_fin
+  double* hfin = nullptr;
+  bool launched = false;
+  std::vector<BItem*> act_list, fin_list;
+};
+
 }  // namespace pgp
 
 struct pgp_batcher {
-  hipStream_t s;
-  // finalisation and predict of converged members run on their own stream
-  // (s2), and a large Newton step is split in two interleaved halves, the
-  // second on s3: the three member sets are disjoint, and the latency-bound
-  // launches of one overlap the others' work
-  hipStream_t s2 = nullptr, s3 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
   pgp_fit_options_t opts;
   int refine;
   int cap;
-  Prob** arr[pgp::BATCH_ARRAYS];
-  double* gather;      // per-member step norms (device)
-  double* gather_fin;  // 8 doubles per finalised member (device)
-  uint32_t* ticket;   // start-order ticket of launches on s
-  uint32_t* ticket2;  // ... on s2, and on s3 (separate cache lines)
-  uint32_t* ticket3;
-  Prob* ddesc;            // device staging of descriptors (2 * cap)
-  Prob* hdesc = nullptr;  // pinned host staging of descriptors (2 * cap)
-  // asynchronous launch / finish
-  double* hstep = nullptr;  // pinned copies of gather / gather_fin
-  double* hfin = nullptr;
-  hipEvent_t ev = nullptr;
-  bool launched = false;
-  std::vector<pgp::BItem*> act_list, fin_list;
-  std::vector<double> hg;
-  std::vector<pgp::BItem> items;
+  int lane_np;  // members with Np above this take lane 1 (0: one lane)
+  pgp::Lane lane[pgp::NLANES];
+  std::list<pgp::BItem> items;  // stable addresses: lanes hold pointers
   std::vector<pgp::BItem> pending;
-  std::vector<int> hnp[pgp::BATCH_ARRAYS];
-  std::vector<int> hm[pgp::BATCH_ARRAYS];
 };
 
 namespace pgp {
 
-static int64_t batcher_table_bytes(int cap) {
+static int64_t lane_table_bytes(int cap) {
   return align256((int64_t)BATCH_ARRAYS * cap * 8) + 2 * align256((int64_t)cap * 8 * 8) + 256 +
          align256((int64_t)2 * cap * (int64_t)sizeof(Prob));
 }
+static int64_t batcher_table_bytes(int cap) { return NLANES * lane_table_bytes(cap); }
+
+static bool on_s2(int which) { return which == 3 || which == 4 || which == 5 || which == 7; }
 
 // Write the descriptors of a batch's members (held in v[i]->p) to their
 // workspaces with one staged copy and one kernel.  `region` (0 or 1) picks
-// the half of the staging buffers, so that two writes of one step do not
-// overlap; `arr` is the batch's device pointer array (already written).
-static void write_probs(pgp_batcher* bt, const std::vector<BItem*>& v, Prob* const* arr,
-                        int region, int reset, hipStream_t s) {
+// the half of the lane's staging buffers, so that two writes of one step do
+// not overlap; `arr` is the batch's device pointer array (already written).
+static void write_probs(pgp_batcher* bt, Lane& L, const std::vector<BItem*>& v,
+                        Prob* const* arr, int region, int reset, hipStream_t s) {
   if (v.empty()) return;
-  Prob* h = bt->hdesc + (size_t)region * bt->cap;
-  Prob* d = bt->ddesc + (size_t)region * bt->cap;
+  Prob* h = L.hdesc + (size_t)region * bt->cap;
+  Prob* d = L.ddesc + (size_t)region * bt->cap;
   for (size_t i = 0; i < v.size(); ++i) h[i] = v[i]->p;
   (void)hipMemcpyAsync(d, h, v.size() * sizeof(Prob), hipMemcpyHostToDevice, s);
   hipLaunchKernelGGL(k_scatter_probs, dim3((unsigned)v.size()), dim3(256), 0, s, arr, d,
                      (int)v.size(), reset);
 }
 
-
 // Members are ordered by padded size, largest first (stable), so that the
 // members still active at any block column form a prefix of the batch.
-// Arrays 3..5 belong to the finalisation stream s2, 6..7 to the second
-// Newton half on s3, the others to s.
-// Newton steps of at least this many members are split over two streams
-// when the options ask for it (PGP_FIT_SPLIT_STEPS).
-constexpr int SPLIT_MIN = 16;
-
-static int lane_of(int which) {  // 0: s, 1: s2, 2: s3
-  static const int lane[BATCH_ARRAYS] = {0, 0, 0, 1, 1, 1, 2, 2, 0, 2, 1};
-  return lane[which];
-}
-static hipStream_t stream_of(const pgp_batcher* bt, int which) {
-  const int l = lane_of(which);
-  return l == 2 ? bt->s3 : (l == 1 ? bt->s2 : bt->s);
-}
-
-static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
+static Batch make_batch(Lane& L, int which, std::vector<BItem*>& v) {
   std::stable_sort(v.begin(), v.end(),
                    [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
   std::vector<Prob*> ptrs;
   ptrs.reserve(v.size());
-  std::vector<int>& hnp = bt->hnp[which];
-  std::vector<int>& hm = bt->hm[which];
+  std::vector<int>& hnp = L.hnp[which];
+  std::vector<int>& hm = L.hm[which];
   hnp.clear();
   hm.clear();
   int maxNp = 0;
@@ -1026,53 +1021,52 @@ static Batch make_batch(pgp_batcher* bt, int which, std::vector<BItem*>& v) {
     hm.push_back(b->p.M);
     maxNp = std::max(maxNp, (int)b->p.Np);
   }
-  write_ptrs(bt->arr[which], ptrs, stream_of(bt, which));
+  write_ptrs(L.arr[which], ptrs, on_s2(which) ? L.s2 : L.s);
   Batch b;
-  b.dprobs = bt->arr[which];
+  b.dprobs = L.arr[which];
   b.count = (int)v.size();
   b.maxNp = maxNp;
   b.hnp = hnp.data();
   b.hm = hm.data();
-  const int l = lane_of(which);
-  b.ticket = l == 2 ? bt->ticket3 : (l == 1 ? bt->ticket2 : bt->ticket);
+  b.ticket = on_s2(which) ? L.ticket2 : L.ticket;
   return b;
 }
 
-// Admit pending requests up to capacity; new members get their descriptor,
-// Gram matrix and a zero weight.
+// Move pending requests into the batch up to capacity and give each a lane.
 static void admit(pgp_batcher* bt) {
-  hipStream_t s = bt->s;
   size_t take = 0;
   while (take < bt->pending.size() && (int)bt->items.size() < bt->cap) {
-    bt->items.push_back(bt->pending[take]);
-    ++take;
+    BItem b = bt->pending[take++];
+    b.lane = (bt->lane_np > 0 && b.p.Np > bt->lane_np) ? 1 : 0;
+    bt->items.push_back(b);
   }
   bt->pending.erase(bt->pending.begin(), bt->pending.begin() + take);
-  std::vector<BItem*> fresh;
-  for (BItem& b : bt->items)
-    if (b.phase == 0) fresh.push_back(&b);
-  if (!fresh.empty()) {
-    const Batch bn = make_batch(bt, 0, fresh);
-    write_probs(bt, fresh, bn.dprobs, 0, 1, s);
-    launch_gram_self(bn, s);
-    launch_vec_op(bn, OP_ZERO_Z, s);
-    for (BItem* b : fresh) b->phase = 1;
-  }
 }
 
-// Enqueue on s2 the final state (f, W, log p, B factored at W) and the
-// predict of the members in `conv`; 8 result doubles per member land in
-// `gather` and are copied to the pinned hfin.
-static void finalize_and_predict(pgp_batcher* bt, std::vector<BItem*>& conv, double* gather) {
-  hipStream_t s = bt->s2;
+// New members of `fresh` get their descriptor, Gram matrix and a zero
+// weight (on the lane's Newton stream).
+static void start_members(pgp_batcher* bt, Lane& L, std::vector<BItem*>& fresh) {
+  if (fresh.empty()) return;
+  const Batch bn = make_batch(L, 0, fresh);
+  write_probs(bt, L, fresh, bn.dprobs, 0, 1, L.s);
+  launch_gram_self(bn, L.s);
+  launch_vec_op(bn, OP_ZERO_Z, L.s);
+  for (BItem* b : fresh) b->phase = 1;
+}
+
+// Enqueue on the lane's finalisation stream the final state (f, W, log p,
+// B factored at W) and the predict of the members in `conv`; 8 result
+// doubles per member land in the lane's gather_fin, copied to its hfin.
+static void finalize_and_predict(pgp_batcher* bt, Lane& L, std::vector<BItem*>& conv) {
+  hipStream_t s = L.s2;
   const int prec = bt->opts.precision;
   std::vector<BItem*> conv_fac, conv_fac32, pred;
   for (BItem* b : conv)
     if (!b->gauss) (b->esc ? conv_fac32 : conv_fac).push_back(b);
-  const Batch bc = make_batch(bt, 3, conv);
-  const Batch bcf = make_batch(bt, 4, conv_fac);
-  const Batch bcf32 = make_batch(bt, 10, conv_fac32);
-  laplace_finalize(bc, &bcf, &bcf32, prec, gather, s);
+  const Batch bc = make_batch(L, 3, conv);
+  const Batch bcf = make_batch(L, 4, conv_fac);
+  const Batch bcf32 = make_batch(L, 7, conv_fac32);
+  laplace_finalize(bc, &bcf, &bcf32, prec, L.gather_fin, s);
   int maxM = 0;
   for (BItem* b : conv) {
     pgp_batch_item_t* it = b->item;
@@ -1090,14 +1084,14 @@ static void finalize_and_predict(pgp_batcher* bt, std::vector<BItem*>& conv, dou
     maxM = std::max(maxM, (int)it->M);
   }
   if (!pred.empty()) {
-    const Batch bp = make_batch(bt, 5, pred);
-    write_probs(bt, pred, bp.dprobs, 1, 0, s);
+    const Batch bp = make_batch(L, 5, pred);
+    write_probs(bt, L, pred, bp.dprobs, 1, 0, s);
     const int maxMp = (int)round_up(maxM, TILE);
     launch_cross_prep(bp, prec, maxM, s);
     launch_trsm(bp, prec, maxMp, 0, s);
     launch_predict_final(bp, prec, maxM, s);
   }
-  (void)hipMemcpyAsync(bt->hfin, gather, conv.size() * 64, hipMemcpyDeviceToHost, s);
+  (void)hipMemcpyAsync(L.hfin, L.gather_fin, conv.size() * 64, hipMemcpyDeviceToHost, s);
 }
 
 // Results of finalised members from their 8 gathered doubles each.
@@ -1119,13 +1113,9 @@ static void report(pgp_batcher* bt, const std::vector<BItem*>& conv, const doubl
   }
 }
 
-// Drop finished members (stable order).
+// Drop finished members (the others keep their addresses).
 static void drop_done(pgp_batcher* bt) {
-  std::vector<BItem> keep;
-  keep.reserve(bt->items.size());
-  for (BItem& b : bt->items)
-    if (!b.done) keep.push_back(b);
-  bt->items.swap(keep);
+  bt->items.remove_if([](const BItem& b) { return b.done; });
 }
 
 // BF16 batcher, as the single-problem fit: a member whose factor lost
@@ -1151,23 +1141,109 @@ __global__ void k_clear_not_pd(Prob* const* arr, int n) {
   if (i < n) arr[i]->info[0] = 0;
 }
 
-// Clear the flags of members escalated in the previous step (array 0, after
-// the admission's launches on the same stream).
-static void clear_escalated(pgp_batcher* bt) {
+// Clear the flags of the members of `v` escalated in their previous step
+// (array 0, after the admission's launches on the same stream).
+static void clear_escalated(Lane& L, const std::vector<BItem*>& v) {
   std::vector<BItem*> clr;
-  for (BItem& b : bt->items)
-    if (b.clear) {
-      clr.push_back(&b);
-      b.clear = false;
+  for (BItem* b : v)
+    if (b->clear) {
+      clr.push_back(b);
+      b->clear = false;
     }
   if (clr.empty()) return;
-  const Batch bz = make_batch(bt, 0, clr);
-  hipLaunchKernelGGL(k_clear_not_pd, dim3((bz.count + 255) / 256), dim3(256), 0, bt->s, bz.dprobs,
+  const Batch bz = make_batch(L, 0, clr);
+  hipLaunchKernelGGL(k_clear_not_pd, dim3((bz.count + 255) / 256), dim3(256), 0, L.s, bz.dprobs,
                      bz.count);
 }
 
 static bool converged(const pgp_batcher* bt, const BItem* b) {
   return b->step <= bt->opts.tolerance || b->iter >= bt->opts.maxiter || !(b->step == b->step);
+}
+
+// Step norm of a member after its Newton iteration; true if it converged.
+static bool advance(pgp_batcher* bt, BItem* b, double h2) {
+  b->step = sqrt(h2);
+  if (escalate(bt, b)) return false;
+  b->iter++;
+  b->prev = b->step;
+  return converged(bt, b);
+}
+
+// Enqueue one step of lane `li` (finalisation of the members that converged
+// in its previous step beside one Newton iteration of the others) and record
+// its completion event.  Nothing is enqueued if the lane has no members.
+static int lane_launch(pgp_batcher* bt, int li) {
+  Lane& L = bt->lane[li];
+  hipStream_t s = L.s;
+  const int prec = bt->opts.precision;
+  L.fin_list.clear();
+  L.act_list.clear();
+  std::vector<BItem*> fresh;
+  for (BItem& b : bt->items) {
+    if (b.lane != li) continue;
+    if (b.phase == 0) fresh.push_back(&b);
+    if (b.phase == 2) L.fin_list.push_back(&b);
+  }
+  start_members(bt, L, fresh);
+  for (BItem& b : bt->items)
+    if (b.lane == li && b.phase == 1) L.act_list.push_back(&b);
+  if (L.fin_list.empty() && L.act_list.empty()) return 0;
+  clear_escalated(L, L.act_list);
+  // (make_batch keeps this order: it is already sorted by size)
+  std::stable_sort(L.act_list.begin(), L.act_list.end(),
+                   [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
+  std::vector<BItem*> fac, fac32;
+  for (BItem* b : L.act_list) {
+    if (!b->gauss || b->iter == 0 || b->refac) (b->esc ? fac32 : fac).push_back(b);
+    b->refac = false;
+  }
+  const bool fin = !L.fin_list.empty();
+  if (fin) {
+    // s2 follows everything enqueued on s so far, then runs beside the
+    // Newton step; s joins it before the completion event
+    HIP_OK(hipEventRecord(L.ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(L.s2, L.ev_fork, 0));
+    finalize_and_predict(bt, L, L.fin_list);
+    HIP_OK(hipEventRecord(L.ev_join, L.s2));
+    for (BItem* b : L.fin_list) b->phase = 3;
+  }
+  if (!L.act_list.empty()) {
+    const Batch ba = make_batch(L, 1, L.act_list);
+    const Batch bf = make_batch(L, 2, fac);
+    const Batch bf32 = make_batch(L, 6, fac32);
+    newton_iter(ba, &bf, &bf32, prec, bt->refine, L.gather, s);
+    HIP_OK(hipMemcpyAsync(L.hstep, L.gather, L.act_list.size() * 8, hipMemcpyDeviceToHost, s));
+  }
+  int rc = check_launch();
+  if (rc) return rc;
+  if (fin) HIP_OK(hipStreamWaitEvent(s, L.ev_join, 0));
+  HIP_OK(hipEventRecord(L.ev, s));
+  L.launched = true;
+  return 0;
+}
+
+// Results of lane `li`'s finished step: report its finalised members and
+// advance the others.
+static void lane_finish(pgp_batcher* bt, int li, int64_t* done_tags, int32_t max_done,
+                        int32_t* n_done) {
+  Lane& L = bt->lane[li];
+  L.launched = false;
+  report(bt, L.fin_list, L.hfin, done_tags, max_done, n_done);
+  for (size_t i = 0; i < L.act_list.size(); ++i)
+    if (advance(bt, L.act_list[i], L.hstep[i])) L.act_list[i]->phase = 2;
+  L.fin_list.clear();
+  L.act_list.clear();
+  drop_done(bt);
+}
+
+// Lane split (PGP_LANE_NP, default 1024): members with Np above it take
+// lane 1; 0 keeps every member in lane 0.
+static int lane_np() {
+  static const int v = [] {
+    const char* e = getenv("PGP_LANE_NP");
+    return e ? atoi(e) : 1024;
+  }();
+  return v;
 }
 
 }  // namespace pgp
@@ -1188,36 +1264,45 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
     return fail(-3, "batcher table too small");
   if (!out) return fail(-6, "out is NULL");
   pgp_batcher* b = new pgp_batcher();
-  b->s = (hipStream_t)stream;
   b->opts = *opts;
   b->refine = opts->refine >= 0 ? opts->refine : default_refine(opts->precision);
   b->cap = capacity;
-  char* t = (char*)table;
-  for (int i = 0; i < BATCH_ARRAYS; ++i) b->arr[i] = (Prob**)(t + (int64_t)i * capacity * 8);
-  b->gather = (double*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8));
-  b->gather_fin = (double*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
-                            align256((int64_t)capacity * 8 * 8));
-  b->ticket = (uint32_t*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
-                          2 * align256((int64_t)capacity * 8 * 8));
-  b->ticket2 = b->ticket + 32;
-  b->ticket3 = b->ticket + 48;
-  b->ddesc = (Prob*)(t + align256((int64_t)BATCH_ARRAYS * capacity * 8) +
-                     2 * align256((int64_t)capacity * 8 * 8) + 256);
-  b->hg.resize((size_t)capacity * 8);
-  if (hipMemsetAsync(b->ticket, 0, 256, b->s) != hipSuccess ||
-      hipHostMalloc((void**)&b->hstep, (size_t)capacity * 8, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&b->hfin, (size_t)capacity * 64, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&b->hdesc, (size_t)2 * capacity * sizeof(Prob),
-                    hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_join3, hipEventDisableTiming) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->s3, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamSynchronize(b->s) != hipSuccess) {
+  b->lane_np = lane_np();
+  bool ok = true;
+  for (int li = 0; li < NLANES; ++li) {
+    Lane& L = b->lane[li];
+    char* t = (char*)table + li * lane_table_bytes(capacity);
+    for (int i = 0; i < BATCH_ARRAYS; ++i) L.arr[i] = (Prob**)(t + (int64_t)i * capacity * 8);
+    int64_t off = align256((int64_t)BATCH_ARRAYS * capacity * 8);
+    L.gather = (double*)(t + off);
+    off += align256((int64_t)capacity * 8 * 8);
+    L.gather_fin = (double*)(t + off);
+    off += align256((int64_t)capacity * 8 * 8);
+    L.ticket = (uint32_t*)(t + off);
+    L.ticket2 = L.ticket + 32;
+    off += 256;
+    L.ddesc = (Prob*)(t + off);
+    if (li == 0) {
+      L.s = (hipStream_t)stream;
+    } else {
+      ok = ok && hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking) == hipSuccess;
+      L.own_s = ok;
+    }
+    ok = ok && hipMemsetAsync(L.ticket, 0, 256, (hipStream_t)stream) == hipSuccess &&
+         hipHostMalloc((void**)&L.hstep, (size_t)capacity * 8, hipHostMallocDefault) ==
+             hipSuccess &&
+         hipHostMalloc((void**)&L.hfin, (size_t)capacity * 64, hipHostMallocDefault) ==
+             hipSuccess &&
+         hipHostMalloc((void**)&L.hdesc, (size_t)2 * capacity * sizeof(Prob),
+                       hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&L.ev_fork, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&L.ev_join, hipEventDisableTiming) == hipSuccess &&
+         hipStreamCreateWithFlags(&L.s2, hipStreamNonBlocking) == hipSuccess;
+  }
+  if (!ok || hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
     pgp_batcher_destroy(b);
-    return fail(-100, "batcher set-up (table clear, pinned buffers, events, stream) failed");
+    return fail(-100, "batcher set-up (table clear, pinned buffers, events, streams) failed");
   }
   *out = b;
   return 0;
@@ -1246,6 +1331,7 @@ int pgp_batcher_submit(pgp_batcher_t* bt, pgp_batch_item_t* item, int64_t tag) {
   b.p = make_prob(&item->problem, &item->kernel, &item->lik, prec, b.ws);
   b.phase = 0;
   b.iter = 0;
+  b.lane = 0;
   b.done = false;
   b.step = INFINITY;
   b.gauss = item->lik.kind == PGP_LIK_GAUSSIAN;
@@ -1258,118 +1344,64 @@ int32_t pgp_batcher_in_flight(pgp_batcher_t* bt) {
   return bt ? (int32_t)(bt->items.size() + bt->pending.size()) : 0;
 }
 
+int32_t pgp_batcher_launched(pgp_batcher_t* bt) {
+  if (!bt) return 0;
+  int32_t n = 0;
+  for (int li = 0; li < NLANES; ++li) n += bt->lane[li].launched ? 1 : 0;
+  return n;
+}
+
 int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, int32_t* n_done) {
   if (!bt) return fail(-1, "batcher is NULL");
   if (!n_done) return fail(-4, "n_done is NULL");
   *n_done = 0;
-  if (bt->launched) return fail(-1, "pgp_batcher_step: an asynchronous launch is pending");
-  hipStream_t s = bt->s;
+  if (pgp_batcher_launched(bt)) return fail(-1, "pgp_batcher_step: an asynchronous launch is pending");
+  // every member in one step, on lane 0's streams
+  Lane& L = bt->lane[0];
+  hipStream_t s = L.s;
   const int prec = bt->opts.precision;
   admit(bt);
-  clear_escalated(bt);
-  std::vector<BItem*> active, fac, fac32;
-  for (BItem& b : bt->items) {
-    active.push_back(&b);
-    if (!b.gauss || b.iter == 0 || b.refac) (b.esc ? fac32 : fac).push_back(&b);
-    b.refac = false;
-  }
+  std::vector<BItem*> fresh, active, fac, fac32;
+  for (BItem& b : bt->items)
+    if (b.phase == 0) fresh.push_back(&b);
+  start_members(bt, L, fresh);
+  for (BItem& b : bt->items) active.push_back(&b);
   if (active.empty()) return 0;
-  const Batch ba = make_batch(bt, 1, active);
-  const Batch bf = make_batch(bt, 2, fac);
-  const Batch bf32 = make_batch(bt, 8, fac32);
-  newton_iter(ba, &bf, &bf32, prec, bt->refine, bt->gather, s);
+  clear_escalated(L, active);
+  for (BItem* b : active) {
+    if (!b->gauss || b->iter == 0 || b->refac) (b->esc ? fac32 : fac).push_back(b);
+    b->refac = false;
+  }
+  const Batch ba = make_batch(L, 1, active);
+  const Batch bf = make_batch(L, 2, fac);
+  const Batch bf32 = make_batch(L, 6, fac32);
+  newton_iter(ba, &bf, &bf32, prec, bt->refine, L.gather, s);
   int rc = check_launch();
   if (rc) return rc;
-  HIP_OK(hipMemcpyAsync(bt->hstep, bt->gather, active.size() * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(L.hstep, L.gather, active.size() * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   std::vector<BItem*> conv;
-  for (size_t i = 0; i < active.size(); ++i) {
-    BItem* b = active[i];
-    b->step = sqrt(bt->hstep[i]);
-    if (escalate(bt, b)) continue;
-    b->iter++;
-    b->prev = b->step;
-    if (converged(bt, b)) conv.push_back(b);
-  }
+  for (size_t i = 0; i < active.size(); ++i)
+    if (advance(bt, active[i], L.hstep[i])) conv.push_back(active[i]);
   if (conv.empty()) return 0;
   // s is idle (synchronised above), so s2 starts after the Newton step
-  finalize_and_predict(bt, conv, bt->gather_fin);
+  finalize_and_predict(bt, L, conv);
   rc = check_launch();
   if (rc) return rc;
-  HIP_OK(hipStreamSynchronize(bt->s2));
-  report(bt, conv, bt->hfin, done_tags, max_done, n_done);
+  HIP_OK(hipStreamSynchronize(L.s2));
+  report(bt, conv, L.hfin, done_tags, max_done, n_done);
   drop_done(bt);
   return 0;
 }
 
 int pgp_batcher_launch(pgp_batcher_t* bt) {
   if (!bt) return fail(-1, "batcher is NULL");
-  if (bt->launched) return fail(-1, "pgp_batcher_launch: the previous launch is not finished");
-  hipStream_t s = bt->s;
-  const int prec = bt->opts.precision;
   admit(bt);
-  clear_escalated(bt);
-  bt->fin_list.clear();
-  bt->act_list.clear();
-  for (BItem& b : bt->items) {
-    if (b.phase == 2)
-      bt->fin_list.push_back(&b);
-    else if (b.phase == 1)
-      bt->act_list.push_back(&b);
+  for (int li = 0; li < NLANES; ++li) {
+    if (bt->lane[li].launched) continue;
+    const int rc = lane_launch(bt, li);
+    if (rc) return rc;
   }
-  // a large step is split in two halves of alternating size rank, so both
-  // carry the same mix of sizes; act_list becomes half A then half B (the
-  // order of the gathered step norms)
-  // (make_batch keeps this order: it is already sorted by size)
-  std::vector<BItem*> half[2], fac[2], fac32[2];
-  std::stable_sort(bt->act_list.begin(), bt->act_list.end(),
-                   [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
-  if ((bt->opts.flags & PGP_FIT_SPLIT_STEPS) && (int)bt->act_list.size() >= SPLIT_MIN) {
-    for (size_t i = 0; i < bt->act_list.size(); ++i) half[i & 1].push_back(bt->act_list[i]);
-    bt->act_list = half[0];
-    bt->act_list.insert(bt->act_list.end(), half[1].begin(), half[1].end());
-  } else {
-    half[0] = bt->act_list;
-  }
-  for (int h = 0; h < 2; ++h)
-    for (BItem* b : half[h])
-      if (!b->gauss || b->iter == 0 || b->refac) (b->esc ? fac32[h] : fac[h]).push_back(b);
-  for (BItem* b : bt->act_list) b->refac = false;
-  const bool fin = !bt->fin_list.empty(), second = !half[1].empty();
-  if (fin || second) {
-    // s2 / s3 follow everything enqueued on s so far, then run beside the
-    // Newton step on s; s joins them before the completion event
-    HIP_OK(hipEventRecord(bt->ev_fork, s));
-    if (fin) HIP_OK(hipStreamWaitEvent(bt->s2, bt->ev_fork, 0));
-    if (second) HIP_OK(hipStreamWaitEvent(bt->s3, bt->ev_fork, 0));
-  }
-  if (fin) {
-    finalize_and_predict(bt, bt->fin_list, bt->gather_fin);
-    HIP_OK(hipEventRecord(bt->ev_join, bt->s2));
-    for (BItem* b : bt->fin_list) b->phase = 3;
-  }
-  if (second) {
-    const Batch ba = make_batch(bt, 6, half[1]);
-    const Batch bf = make_batch(bt, 7, fac[1]);
-    const Batch bf32 = make_batch(bt, 9, fac32[1]);
-    newton_iter(ba, &bf, &bf32, prec, bt->refine, bt->gather + half[0].size(), bt->s3);
-    HIP_OK(hipEventRecord(bt->ev_join3, bt->s3));
-  }
-  if (!half[0].empty()) {
-    const Batch ba = make_batch(bt, 1, half[0]);
-    const Batch bf = make_batch(bt, 2, fac[0]);
-    const Batch bf32 = make_batch(bt, 8, fac32[0]);
-    newton_iter(ba, &bf, &bf32, prec, bt->refine, bt->gather, s);
-  }
-  if (second) HIP_OK(hipStreamWaitEvent(s, bt->ev_join3, 0));
-  if (!bt->act_list.empty())
-    HIP_OK(hipMemcpyAsync(bt->hstep, bt->gather, bt->act_list.size() * 8,
-                          hipMemcpyDeviceToHost, s));
-  int rc = check_launch();
-  if (rc) return rc;
-  if (fin) HIP_OK(hipStreamWaitEvent(s, bt->ev_join, 0));
-  HIP_OK(hipEventRecord(bt->ev, s));
-  bt->launched = true;
   return 0;
 }
 
@@ -1378,39 +1410,43 @@ int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
   if (!bt) return fail(-1, "batcher is NULL");
   if (!n_done) return fail(-4, "n_done is NULL");
   *n_done = 0;
-  if (!bt->launched) return 0;
-  HIP_OK(hipEventSynchronize(bt->ev));
-  bt->launched = false;
-  report(bt, bt->fin_list, bt->hfin, done_tags, max_done, n_done);
-  for (size_t i = 0; i < bt->act_list.size(); ++i) {
-    BItem* b = bt->act_list[i];
-    b->step = sqrt(bt->hstep[i]);
-    if (escalate(bt, b)) continue;
-    b->iter++;
-    b->prev = b->step;
-    if (converged(bt, b)) b->phase = 2;
+  if (!pgp_batcher_launched(bt)) return 0;
+  // wait until at least one launched lane has finished its step
+  for (;;) {
+    bool any = false;
+    for (int li = 0; li < NLANES; ++li) {
+      Lane& L = bt->lane[li];
+      if (!L.launched) continue;
+      const hipError_t e = hipEventQuery(L.ev);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return fail(-100, "lane %d step failed: %s", li, hipGetErrorString(e));
+      lane_finish(bt, li, done_tags, max_done, n_done);
+      any = true;
+    }
+    if (any) return 0;
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
-  bt->fin_list.clear();
-  bt->act_list.clear();
-  drop_done(bt);
-  return 0;
 }
 
 int pgp_batcher_destroy(pgp_batcher_t* bt) {
   if (!bt) return 0;
-  if (bt->launched) (void)hipEventSynchronize(bt->ev);
-  for (hipStream_t x : {bt->s2, bt->s3})
-    if (x) {
-      (void)hipStreamSynchronize(x);
-      (void)hipStreamDestroy(x);
+  for (int li = 0; li < NLANES; ++li) {
+    Lane& L = bt->lane[li];
+    if (L.launched && L.ev) (void)hipEventSynchronize(L.ev);
+    if (L.s2) {
+      (void)hipStreamSynchronize(L.s2);
+      (void)hipStreamDestroy(L.s2);
     }
-  if (bt->ev) (void)hipEventDestroy(bt->ev);
-  if (bt->ev_fork) (void)hipEventDestroy(bt->ev_fork);
-  if (bt->ev_join) (void)hipEventDestroy(bt->ev_join);
-  if (bt->ev_join3) (void)hipEventDestroy(bt->ev_join3);
-  if (bt->hstep) (void)hipHostFree(bt->hstep);
-  if (bt->hfin) (void)hipHostFree(bt->hfin);
-  if (bt->hdesc) (void)hipHostFree(bt->hdesc);
+    if (L.own_s && L.s) {
+      (void)hipStreamSynchronize(L.s);
+      (void)hipStreamDestroy(L.s);
+    }
+    for (hipEvent_t e : {L.ev, L.ev_fork, L.ev_join})
+      if (e) (void)hipEventDestroy(e);
+    if (L.hstep) (void)hipHostFree(L.hstep);
+    if (L.hfin) (void)hipHostFree(L.hfin);
+    if (L.hdesc) (void)hipHostFree(L.hdesc);
+  }
   delete bt;
   return 0;
 }

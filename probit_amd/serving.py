@@ -41,7 +41,7 @@ class ServingEngine:
     """One process, one GPU: a batcher plus an HBM-budget admitter."""
 
     def __init__(self, precision="bf16", capacity=256, hbm_budget_bytes=None, tolerance=1e-5,
-                 maxiter=100, refine=None, device="cuda", split_steps=False):
+                 maxiter=100, refine=None, device="cuda"):
         self.lib = _lib.lib()
         self.device = torch.device(device)
         self.prec = _lib.PRECISIONS[precision]
@@ -56,7 +56,6 @@ class ServingEngine:
         opts.maxiter = maxiter
         opts.precision = self.prec
         opts.refine = -1 if refine is None else int(refine)
-        opts.flags = _lib.FIT_SPLIT_STEPS if split_steps else 0
         self._opts = opts
         tb = self.lib.pgp_batcher_table_size(self.capacity)
         self._table = _lib.workspace(tb, self.device)
@@ -191,22 +190,21 @@ class ServingEngine:
         return out
 
     def launch(self) -> bool:
-        """Enqueue one step for everything in flight and return at once
-        (requests submitted meanwhile join at the next launch).  Returns
-        False if there was nothing to launch."""
-        if not self._slots or self._launched:
-            return False
-        _lib.check(self.lib.pgp_batcher_launch(self._h), "pgp_batcher_launch")
-        self._launched = True
-        return True
+        """Enqueue one step on every lane (small / large requests) that has
+        none in flight and return at once (requests submitted meanwhile join
+        at the next launch).  Returns True if any step is in flight."""
+        if self._slots:
+            _lib.check(self.lib.pgp_batcher_launch(self._h), "pgp_batcher_launch")
+        self._launched = self.lib.pgp_batcher_launched(self._h) > 0
+        return self._launched
 
     def finish(self) -> list[Completed]:
-        """Wait for the launched step; returns the requests it completed."""
+        """Wait until a lane's step is done; returns the requests it completed."""
         if not self._launched:
             return []
         n = C.c_int32(0)
         rc = self.lib.pgp_batcher_finish(self._h, self._done_buf, self.capacity, C.byref(n))
-        self._launched = False
+        self._launched = self.lib.pgp_batcher_launched(self._h) > 0
         _lib.check(rc, "pgp_batcher_finish")
         return self._collect(n.value)
 

@@ -298,16 +298,18 @@ def test_serving_bf16_escalates_like_single_fit():
 @pytest.mark.parametrize("count,capacity", [(4, 3), (40, 32)])
 def test_serving_async_launch_finish_matches_step(count, capacity):
     """The asynchronous launch / finish pair gives bit-identical results to
-    the synchronous step (items are reported one step later).  With 32 in
-    flight the asynchronous Newton steps are split over two streams."""
+    the synchronous step (items are reported one step later).  In the larger
+    case every fifth request has N > 1024 and runs in the second lane, with
+    its own step cadence."""
     from probit_amd.serving import ServingEngine
     shapes = [(150, 2, "probit", "EQ"), (300, 1, "gaussian", "Matern12"),
               (70, 3, "probit", "Matern12"), (260, 2, "gaussian", "EQ")]
     rng = np.random.default_rng(80)
     while len(shapes) < count:
         k = len(shapes)
-        shapes.append((int(rng.integers(40, 400)), int(rng.integers(1, 4)),
-                       ("probit", "gaussian")[k % 2], ("EQ", "Matern12")[k % 3 % 2]))
+        n = int(rng.integers(1100, 1400)) if k % 5 == 0 else int(rng.integers(40, 400))
+        shapes.append((n, int(rng.integers(1, 4)), ("probit", "gaussian")[k % 2],
+                       ("EQ", "Matern12")[k % 3 % 2]))
     probs = []
     for i, (N, D, lik, kind) in enumerate(shapes):
         X, y, theta, lp, spec = small_problem(60 + i, N, D, lik, kind)
@@ -315,8 +317,7 @@ def test_serving_async_launch_finish_matches_step(count, capacity):
         probs.append((X, y, theta, lp, lik, kind, Xt))
 
     def run(async_mode):
-        eng = ServingEngine(precision="fp32", capacity=capacity, hbm_budget_bytes=1 << 32,
-                            split_steps=True)
+        eng = ServingEngine(precision="fp32", capacity=capacity, hbm_budget_bytes=1 << 33)
         out, queue = {}, list(range(count))
         while queue or eng.in_flight:
             launched = eng.launch() if async_mode else False
