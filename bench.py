@@ -298,7 +298,7 @@ def main():
                         "EQ / Matern12, 256 test points; approximate_posterior + predict",
             "requests_per_step": R,
             "capacity": args.capacity,
-            "lane_np": int(os.environ.get("PGP_LANE_NP", "1024")),
+            "lane_np": os.environ.get("PGP_LANE_NP", "1024"),
             "precision": args.precision,
             "tolerance": 1e-5,
         },

@@ -937,7 +937,7 @@ struct BItem {
   Prob p;
   int phase;  // 0 = new, 1 = Newton, 2 = converged (finalise next), 3 = finalising
   int iter;
-  int lane;   // asynchronous form: 0 small members, 1 large ones
+  int lane;   // asynchronous form: size class (0 = smallest)
   double step;
   bool gauss;
   bool done;
@@ -972,7 +972,7 @@ struct pgp_batcher {
   pgp_fit_options_t opts;
   int refine;
   int cap;
-  int lane_np;  // members with Np above this take lane 1 (0: one lane)
+  int lane_np[pgp::NLANES - 1];  // size class bounds on Np (0: unused)
   pgp::Lane lane[pgp::NLANES];
   std::list<pgp::BItem> items;  // stable addresses: lanes hold pointers
   std::vector<pgp::BItem> pending;
@@ -1037,7 +1037,9 @@ static void admit(pgp_batcher* bt) {
   size_t take = 0;
   while (take < bt->pending.size() && (int)bt->items.size() < bt->cap) {
     BItem b = bt->pending[take++];
-    b.lane = (bt->lane_np > 0 && b.p.Np > bt->lane_np) ? 1 : 0;
+    b.lane = 0;
+    for (int k = 0; k < NLANES - 1; ++k)
+      if (bt->lane_np[k] > 0 && b.p.Np > bt->lane_np[k]) b.lane = k + 1;
     bt->items.push_back(b);
   }
   bt->pending.erase(bt->pending.begin(), bt->pending.begin() + take);
@@ -1236,14 +1238,19 @@ static void lane_finish(pgp_batcher* bt, int li, int64_t* done_tags, int32_t max
   drop_done(bt);
 }
 
-// Lane split (PGP_LANE_NP, default 1024): members with Np above it take
-// lane 1; 0 keeps every member in lane 0.
-static int lane_np() {
-  static const int v = [] {
-    const char* e = getenv("PGP_LANE_NP");
-    return e ? atoi(e) : 1024;
-  }();
-  return v;
+// Size-class bounds on the padded N (PGP_LANE_NP: up to NLANES - 1
+// ascending values separated by commas, default "1024"; "0" keeps every
+// member in lane 0).  A member takes the lane of the number of bounds it
+// exceeds.
+static void lane_bounds(int* out) {
+  for (int k = 0; k < NLANES - 1; ++k) out[k] = 0;
+  const char* e = getenv("PGP_LANE_NP");
+  const char* p = e ? e : "1024";
+  for (int k = 0; k < NLANES - 1 && *p; ++k) {
+    out[k] = atoi(p);
+    while (*p && *p != ',') ++p;
+    if (*p == ',') ++p;
+  }
 }
 
 }  // namespace pgp
@@ -1267,7 +1274,7 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
   b->opts = *opts;
   b->refine = opts->refine >= 0 ? opts->refine : default_refine(opts->precision);
   b->cap = capacity;
-  b->lane_np = lane_np();
+  lane_bounds(b->lane_np);
   bool ok = true;
   for (int li = 0; li < NLANES; ++li) {
     Lane& L = b->lane[li];
