@@ -95,36 +95,48 @@ __global__ __launch_bounds__(NTHREADS) void k_symv_tiles(Prob* const* probs, Sch
   if (bx >= nb * (nb + 1) / 2) return;
   int ti, tj;
   lower_tile_v(bx, ti, tj);
-  __shared__ double t[TILE][TILE + 1];
-  __shared__ double xi[TILE], xj[TILE];
-  __shared__ double rpart[4][TILE], cpart[4][TILE];
+  __shared__ double rpart[4][TILE];
   const size_t ld = (size_t)P.Np;
   const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
   const double* x = vslot(P, xs);
   const double* Kt = P.K + (size_t)ti * TILE + (size_t)tj * TILE * ld;
-#pragma unroll 4
-  for (int q = 0; q < 16; ++q) {
-    const int c = w + 4 * q;
-    t[r][c] = Kt[(size_t)r + (size_t)c * ld];  // lane = row: coalesced down a column
-  }
-  if (w == 0) xi[r] = x[ti * TILE + r];
-  if (w == 1) xj[r] = x[tj * TILE + r];
-  __syncthreads();
-  double rs = 0.0, cs = 0.0;
+  // lane r holds row r of the wave's 16 columns c = w + 4q (coalesced down
+  // each column); the tile never goes through LDS
+  double k[16];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int k = 16 * w + u;
-    rs += t[r][k] * xj[k];  // row r, columns 16w..
-    cs += t[k][r] * xi[k];  // column r, rows 16w..
-  }
+  for (int q = 0; q < 16; ++q) k[q] = Kt[(size_t)r + (size_t)(w + 4 * q) * ld];
+  const double xi = x[ti * TILE + r];
+  double rs = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) rs += k[q] * x[tj * TILE + w + 4 * q];
   rpart[w][r] = rs;
-  cpart[w][r] = cs;
-  __syncthreads();
   double* part = P.symv;
+  if (ti != tj) {
+    // column sums K_IJ^T x_I of the wave's 16 columns: a transposing
+    // butterfly halves the values per lane at each of lane bits 5..2, then
+    // bits 1, 0 finish the sum; lane l ends with column q = (l >> 2) & 15
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = k[q] * xi;
+#pragma unroll
+    for (int h = 8, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+      const bool up = (r & bit) != 0;
+#pragma unroll
+      for (int q = 0; q < h; ++q) {
+        const double give = up ? v[q] : v[q + h];
+        const double keep = up ? v[q + h] : v[q];
+        v[q] = keep + __shfl_xor(give, bit);
+      }
+    }
+    double cs = v[0];
+    cs += __shfl_xor(cs, 2);
+    cs += __shfl_xor(cs, 1);
+    if ((r & 3) == 0)
+      part[((size_t)tj * nb + ti) * TILE + w + 4 * ((r >> 2) & 15)] = cs;
+  }
+  __syncthreads();
   if (w == 0)
     part[((size_t)ti * nb + tj) * TILE + r] = (rpart[0][r] + rpart[1][r]) + (rpart[2][r] + rpart[3][r]);
-  else if (w == 1 && ti != tj)
-    part[((size_t)tj * nb + ti) * TILE + r] = (cpart[0][r] + cpart[1][r]) + (cpart[2][r] + cpart[3][r]);
 }
 
 __global__ __launch_bounds__(NTHREADS) void k_symv_reduce(Prob* const* probs, Sched sc, int xs,
