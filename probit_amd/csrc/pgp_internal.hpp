@@ -152,6 +152,19 @@ inline void sched_launch(const Batch& b, Units units, Go go) {
 constexpr int SPIN_LIMIT = 1 << 22;
 uint32_t next_epoch();  // fresh epoch per launch (never 0)
 
+// The same pointer in the global address space.  Loads through a generic
+// pointer compile to FLAT instructions, which count against the LDS wait
+// counter too: a kernel that prefetches from global memory while it reads
+// LDS would then wait for the prefetch at its first LDS read.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p) {
+  return (__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
 // Values handed between workgroups of one launch travel only through
 // relaxed agent-scope accesses: the stores write through the XCD's L2 and
 // the loads read past it, so a publication needs no L2 write-back or
