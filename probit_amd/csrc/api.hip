@@ -72,6 +72,7 @@ constexpr int64_t OFF_RED = 2176;   // double[NRED]
 constexpr int64_t OFF_GATHER = 2432;  // double[8] scratch for reductions
 constexpr int64_t OFF_PROGRESS = 2496;  // uint64 progress word of the wavefront solves
 constexpr int64_t OFF_TICKET = 2504;    // uint32 workgroup-order counter (single-problem launches)
+constexpr int64_t OFF_PROGRESS_CT = 2560;  // uint64[4] progress words of the wavefront TRSM
 constexpr int64_t HEADER = 4096;
 
 struct WsLayout {
@@ -148,6 +149,7 @@ __global__ void k_write_prob(Prob* dst, Prob val, int reset) {
     if (threadIdx.x < NRED) val.red[threadIdx.x] = 0.0;
     // progress word and ticket counter of the header
     if (threadIdx.x < 2) reinterpret_cast<uint64_t*>((char*)dst + OFF_PROGRESS)[threadIdx.x] = 0;
+    if (threadIdx.x < 4) reinterpret_cast<uint64_t*>((char*)dst + OFF_PROGRESS_CT)[threadIdx.x] = 0;
   }
 }
 
@@ -177,6 +179,7 @@ __global__ void k_scatter_probs(Prob* const* dst, const Prob* src, int n, int re
     if (threadIdx.x < 4) v.info[threadIdx.x] = 0;
     if (threadIdx.x < NRED) v.red[threadIdx.x] = 0.0;
     if (threadIdx.x == 0) v.progress[0] = 0;
+    if (threadIdx.x < 4) v.progress_ct[threadIdx.x] = 0;
   }
 }
 
@@ -281,6 +284,7 @@ static Prob make_prob(const pgp_problem_t* pr, const pgp_kernel_t* k, const pgp_
   p.red = (double*)(ws + OFF_RED);
   p.info = (int32_t*)(ws + OFF_INFO);
   p.progress = (uint64_t*)(ws + OFF_PROGRESS);
+  p.progress_ct = (uint64_t*)(ws + OFF_PROGRESS_CT);
   p.out_weight = pr->weight;
   p.out_prec = pr->precision;
   p.out_latent = pr->latent;

@@ -86,6 +86,8 @@ struct Prob {
   uint64_t* progress;
   // symmetric products: per-tile partial sums, (Np/64) x (Np/64) x 64
   double* symv;
+  // wavefront TRSM: one progress word per right-hand-side column tile (4)
+  uint64_t* progress_ct;
 };
 
 __host__ __device__ inline double* vslot(const Prob& p, int v) {

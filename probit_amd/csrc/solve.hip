@@ -453,6 +453,101 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm_lagged_raw(const T* L, int64_
   trsm_step<T>(L, ld, dinv, k, S, lds, blockIdx.x, blockIdx.y, lower_rhs);
 }
 
+// Wavefront TRSM for narrow right-hand sides (at most WAVE_NCT column
+// tiles), all members in one launch: one workgroup per (block row i,
+// member, column tile ct), numbered row-major so that the wave front moves
+// through every member at once.  Workgroup (i, ct) accumulates L_ij X_j[:, ct]
+// for every block j < i as soon as (j, ct) publishes, then forms
+// X_i = DINV_i (S_i - acc) and publishes it: X_i through agent-scope stores,
+// a wait for their completion, then the member's progress word of column
+// tile ct.  Start-order tickets and bounded waits as in k_trsv_wave.
+constexpr int WAVE_NCT = 4;
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_trsm_rows(Prob* const* probs, Sched rows, int nct,
+                                                        uint32_t* ticket, uint32_t epoch) {
+  constexpr int PP = KP<T>::value;  // A operand: conflict-free MFMA reads
+  constexpr int PB = TILE + 1;       // B operand: written transposed, conflict-free stores
+  __shared__ T As[TILE * PP];
+  __shared__ T Bs[TILE * PB];
+  __shared__ int s_val;
+  typedef typename Acc<T>::type acc_t;
+  const int tk = take_ticket(ticket, &s_val);
+  if ((unsigned)tk >= gridDim.x) return;  // counter not zero at launch: no block to take
+  int local;
+  const int i = sched_member(rows, tk, local);
+  const int m = local / nct, ct = local - m * nct;
+  const Prob& P = *probs[m];
+  if (ct >= P.Mp / TILE || i >= P.Np / TILE) return;
+  uint64_t* prog = P.progress_ct + ct;
+  const T* L = (const T*)P.L;
+  T* S = (T*)P.S;
+  const int64_t ld = P.Np;
+  const int64_t ri = (int64_t)i * TILE, n0 = (int64_t)ct * TILE;
+  const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
+  acc_t acc[2][2];
+  zero_acc<T>(acc);
+  int done = 0, ready = 0;
+  while (done < i) {
+    if (done >= ready) {
+      __syncthreads();  // everyone has read the previous s_val
+      if (threadIdx.x == 0) s_val = wait_progress(prog, epoch, done + 1, P.info);
+      __syncthreads();
+      ready = min(__builtin_amdgcn_readfirstlane(s_val), i);
+    }
+    for (; done < ready; ++done) {
+      const int64_t rj = (int64_t)done * TILE;
+      __syncthreads();  // the previous product is done with LDS
+      load_block_kmajor<T, T>(As, PP, L, ld, ri, rj);
+      // B[kk][n] = X_j(kk, n), published in this launch: coherent loads
+#pragma unroll 4
+      for (int q = 0; q < 16; ++q) {
+        const int c = cb + 4 * q;
+        Bs[r * PB + c] = __hip_atomic_load(&S[(size_t)(rj + r) + (size_t)(n0 + c) * ld],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+      mma64_kmajor<T>(As, PP, Bs, PB, acc);
+    }
+  }
+  // X_i = DINV_i (S_i - acc)
+  __syncthreads();
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = acc_row<T>(bm, q), col = acc_col(bn);
+        Bs[row * PB + col] = S[(size_t)(ri + row) + (size_t)(n0 + col) * ld] - acc[bm][bn][q];
+      }
+  load_block_kmajor<double, T>(As, PP, P.dinv + (size_t)i * TILE * TILE, TILE, 0, 0);
+  __syncthreads();
+  zero_acc<T>(acc);
+  mma64_kmajor<T>(As, PP, Bs, PB, acc);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        __hip_atomic_store(&S[(size_t)(ri + acc_row<T>(bm, q)) + (size_t)(n0 + acc_col(bn)) * ld],
+                           (T)acc[bm][bn][q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  publish_wait();  // every wave's X_i stores complete ...
+  __syncthreads();
+  if (threadIdx.x == 0)  // ... before (i + 1, ct) may read them
+    __hip_atomic_store(prog, ((uint64_t)epoch << 32) | (uint64_t)(i + 1), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+static bool wavefront_trsm() {
+  static const bool v = [] {
+    const char* e = getenv("PGP_TRSM_WAVEFRONT");
+    return !(e && e[0] == '0');
+  }();
+  return v;
+}
+
 // Super-steps of G block rows.
 // One launch per super-step q of block rows SS_q = [Gq, Gq + G), with
 // P = SS_{q-1} the super-step finished by the previous launch:
@@ -639,6 +734,41 @@ static void launch_trsm_lagged(const Batch& bt, int prec, int maxMp, int lower_r
 constexpr int TRSM_WIDE = 16;
 
 void launch_trsm(const Batch& bt, int prec, int maxMp, int lower_rhs, hipStream_t s) {
+  const int nct = maxMp / TILE;
+  if (!lower_rhs && bt.ticket && nct <= WAVE_NCT && bt.maxNp / TILE <= MAXB && wavefront_trsm()) {
+    // row-major schedule: row i holds the members with more than i block
+    // rows (a prefix: members are sorted by size), nct workgroups each
+    const int nb = bt.maxNp / TILE;
+    Sched rows;
+    rows.count = nb;
+    int tot = 0;
+    double fl = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      rows.pre[i] = tot;
+      for (int m = 0; m < bt.count; ++m)
+        if (bt.np(m) / TILE > i) tot += nct;
+    }
+    rows.pre[nb] = tot;
+    if (tot == 0) return;
+    const bool on = timer_on(TC_TRSM);
+    if (on) {
+      for (int m = 0; m < bt.count; ++m) {
+        const double np = bt.np(m);
+        const double mp = bt.hm ? (bt.hm[m] + TILE - 1) / TILE * TILE : maxMp;
+        fl += np * np * mp;
+      }
+      timer_begin(TC_TRSM, s);
+    }
+    const uint32_t ep = next_epoch();
+    if (prec == PGP_PREC_FP64)
+      hipLaunchKernelGGL(k_trsm_rows<double>, dim3(tot), dim3(NTHREADS), 0, s, bt.dprobs, rows,
+                         nct, bt.ticket, ep);
+    else
+      hipLaunchKernelGGL(k_trsm_rows<float>, dim3(tot), dim3(NTHREADS), 0, s, bt.dprobs, rows,
+                         nct, bt.ticket, ep);
+    if (on) timer_end(TC_TRSM, s, fl, 0.0);
+    return;
+  }
   if (maxMp / TILE < TRSM_WIDE) {
     launch_trsm_lagged(bt, prec, maxMp, lower_rhs, s);
     return;
