@@ -113,12 +113,15 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv(Prob* const* probs, Sched sc,
 // release semantics after x_i, read with acquire semantics; x travels through
 // agent-scope (L2-coherent) loads and stores.  Workgroups take their block
 // from a ticket counter in start order, so a workgroup only ever waits for
-// workgroups that are already running: the wait always ends.  A wait is
+// workgroups that are already running: the wait always ends.  Tickets run
+// step-major (`rows`: step s holds the members with more than s blocks, a
+// prefix of the size-sorted batch), so the wave front moves through every
+// member at once instead of member after member.  A wait is
 // also bounded (SPIN_LIMIT polls); if the bound is ever reached the member
 // is flagged PGP_INFO_STALLED and the sweep runs on, so every workgroup exits.
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sched sc,
+__global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sched rows,
                                                         uint32_t* ticket, uint32_t epoch,
                                                         int trans, int in, int out) {
   __shared__ double dk[TILE * (TILE + 1)];  // DINV_i, column-major, column pitch 65
@@ -133,9 +136,11 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sche
   __syncthreads();
   const int tk = __builtin_amdgcn_readfirstlane(s_val);
   if ((unsigned)tk >= gridDim.x) return;  // counter not zero at launch: no block to take
-  int local;
-  const Prob& P = *probs[sched_member(sc, tk, local)];
+  int m;
+  const int local = sched_member(rows, tk, m);  // step of this member's sweep
+  const Prob& P = *probs[m];
   const int nb = P.Np / TILE;
+  if (local >= nb) return;
   const int i = trans ? nb - 1 - local : local;
   const int64_t ld = P.Np;
   const T* L = (const T*)P.L;
@@ -277,33 +282,41 @@ uint32_t next_epoch() {
 }
 
 void launch_trsv(const Batch& bt, int prec, int trans, int in, int out, hipStream_t s) {
-  if (bt.ticket && wavefront_trsv()) {
+  if (bt.ticket && wavefront_trsv() && bt.maxNp / TILE <= MAXB) {
+    // step-major schedule: step s holds the members with more than s blocks
+    // (a prefix: members are sorted by size), one workgroup each
+    const int nb = bt.maxNp / TILE;
+    Sched rows;
+    rows.count = nb;
+    int tot = 0;
+    for (int st = 0; st < nb; ++st) {
+      rows.pre[st] = tot;
+      for (int m = 0; m < bt.count; ++m)
+        if (bt.np(m) / TILE > st) ++tot;
+    }
+    rows.pre[nb] = tot;
+    if (tot == 0) return;
     const bool on = timer_on(TC_TRSV);
     const double esz = prec == PGP_PREC_FP64 ? 8.0 : 4.0;
-    sched_launch(
-        bt, [&](int m) { return bt.np(m) / TILE; },
-        [&](Prob* const* dp, const Sched& sc, int n) {
-          const uint32_t ep = next_epoch();
-          // algorithmic work: Np^2 flops; bytes: the lower triangle of L and
-          // the diagonal-block inverses, read once
-          double fl = 0.0, by = 0.0;
-          if (on) {
-            const int off = (int)(dp - bt.dprobs);
-            for (int m = off; m < off + sc.count; ++m) {
-              const double np = bt.np(m);
-              fl += np * np;
-              by += 0.5 * np * (np + TILE) * esz + np * TILE * 8.0;
-            }
-            timer_begin(TC_TRSV, s);
-          }
-          if (prec == PGP_PREC_FP64)
-            hipLaunchKernelGGL(k_trsv_wave<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc,
-                               bt.ticket, ep, trans, in, out);
-          else
-            hipLaunchKernelGGL(k_trsv_wave<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc,
-                               bt.ticket, ep, trans, in, out);
-          if (on) timer_end(TC_TRSV, s, fl, by);
-        });
+    // algorithmic work: Np^2 flops; bytes: the lower triangle of L and the
+    // diagonal-block inverses, read once
+    double fl = 0.0, by = 0.0;
+    if (on) {
+      for (int m = 0; m < bt.count; ++m) {
+        const double np = bt.np(m);
+        fl += np * np;
+        by += 0.5 * np * (np + TILE) * esz + np * TILE * 8.0;
+      }
+      timer_begin(TC_TRSV, s);
+    }
+    const uint32_t ep = next_epoch();
+    if (prec == PGP_PREC_FP64)
+      hipLaunchKernelGGL(k_trsv_wave<double>, dim3(tot), dim3(NTHREADS), 0, s, bt.dprobs, rows,
+                         bt.ticket, ep, trans, in, out);
+    else
+      hipLaunchKernelGGL(k_trsv_wave<float>, dim3(tot), dim3(NTHREADS), 0, s, bt.dprobs, rows,
+                         bt.ticket, ep, trans, in, out);
+    if (on) timer_end(TC_TRSV, s, fl, by);
     return;
   }
   const int nb = bt.maxNp / TILE;
