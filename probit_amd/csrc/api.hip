@@ -315,8 +315,8 @@ static void set_self(char* ws, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Newton pieces (batched)
 // ---------------------------------------------------------------------------
-static void factor(const Batch& b, int prec, int vb, hipStream_t s) {
-  launch_form_B(b, (prec << 8) | vb, s);
+static void factor(const Batch& b, int prec, int vb, hipStream_t s, int rs = -1) {
+  launch_form_B(b, (prec << 8) | vb, s, rs);
   launch_potrf(b, prec, s);
   launch_dinv(b, prec, s);
 }
@@ -329,14 +329,18 @@ static void solve_B(const Batch& b, int prec, hipStream_t s) {
 // One Newton iteration z <- z + (I + W K)^-1 (grad - z) for every member of
 // `all`; members of `fac` re-form and re-factor B first (those of `fac32`
 // with FP32 updates: BF16 members escalated after a loss of definiteness;
-// both store the factor in FP32).  ||delta||^2 goes to gather[b].
-static void newton_iter(const Batch& all, const Batch* fac, const Batch* fac32, int prec,
-                        int refine, double* gather, hipStream_t s) {
+// both store the factor in FP32), and `nofac` lists the others.  The pass
+// that forms B also takes the partial sums of K r, so only the members of
+// `nofac` read K once more for it.  ||delta||^2 goes to gather[b].
+static void newton_iter(const Batch& all, const Batch* fac, const Batch* fac32,
+                        const Batch* nofac, int prec, int refine, double* gather,
+                        hipStream_t s) {
   launch_gemv_sym(all, V_Z, V_F, 0, s);
   launch_lik_newton(all, s);
-  if (fac && fac->count > 0) factor(*fac, prec, 0, s);
-  if (fac32 && fac32->count > 0) factor(*fac32, PGP_PREC_FP32, 0, s);
-  launch_gemv_sym(all, V_R, V_T, 1, s);
+  if (fac && fac->count > 0) factor(*fac, prec, 0, s, V_R);
+  if (fac32 && fac32->count > 0) factor(*fac32, PGP_PREC_FP32, 0, s, V_R);
+  if (nofac && nofac->count > 0) launch_symv_tiles(*nofac, V_R, s);
+  launch_symv_reduce(all, V_R, V_T, 1, s);
   solve_B(all, prec, s);
   launch_vec_op(all, OP_STEP, s);
   for (int r = 0; r < refine; ++r) {
@@ -594,7 +598,8 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   while (step > opts->tolerance && it < opts->maxiter) {
     const bool refactor = !gauss || it == 0 || force_factor;
     force_factor = false;
-    newton_iter(b, refactor ? &b : nullptr, nullptr, cur, refine, gather, s);
+    newton_iter(b, refactor ? &b : nullptr, nullptr, refactor ? nullptr : &b, cur, refine,
+                gather, s);
     rc = check_launch();
     if (rc) return rc;
     double h2;
@@ -930,8 +935,9 @@ namespace pgp {
 // Device pointer arrays of a lane: 0 admitted / flag clearing, 1 Newton
 // members, 2 their factored ones, 3 finalised members, 4 their factored
 // ones, 5 predicted ones, 6 and 7 the FP32-escalated factored members of 2
-// and 4.  Arrays 3, 4, 5 and 7 belong to the lane's finalisation stream.
-constexpr int BATCH_ARRAYS = 8;
+// and 4, 8 the Newton members not re-factored.  Arrays 3, 4, 5 and 7 belong
+// to the lane's finalisation stream.
+constexpr int BATCH_ARRAYS = 9;
 constexpr int NLANES = 2;
 
 struct BItem {
@@ -1198,9 +1204,12 @@ static int lane_launch(pgp_batcher* bt, int li) {
   // (make_batch keeps this order: it is already sorted by size)
   std::stable_sort(L.act_list.begin(), L.act_list.end(),
                    [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
-  std::vector<BItem*> fac, fac32;
+  std::vector<BItem*> fac, fac32, nofac;
   for (BItem* b : L.act_list) {
-    if (!b->gauss || b->iter == 0 || b->refac) (b->esc ? fac32 : fac).push_back(b);
+    if (!b->gauss || b->iter == 0 || b->refac)
+      (b->esc ? fac32 : fac).push_back(b);
+    else
+      nofac.push_back(b);
     b->refac = false;
   }
   const bool fin = !L.fin_list.empty();
@@ -1217,7 +1226,8 @@ static int lane_launch(pgp_batcher* bt, int li) {
     const Batch ba = make_batch(L, 1, L.act_list);
     const Batch bf = make_batch(L, 2, fac);
     const Batch bf32 = make_batch(L, 6, fac32);
-    newton_iter(ba, &bf, &bf32, prec, bt->refine, L.gather, s);
+    const Batch bn = make_batch(L, 8, nofac);
+    newton_iter(ba, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
     HIP_OK(hipMemcpyAsync(L.hstep, L.gather, L.act_list.size() * 8, hipMemcpyDeviceToHost, s));
   }
   int rc = check_launch();
@@ -1372,7 +1382,7 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   hipStream_t s = L.s;
   const int prec = bt->opts.precision;
   admit(bt);
-  std::vector<BItem*> fresh, active, fac, fac32;
+  std::vector<BItem*> fresh, active, fac, fac32, nofac;
   for (BItem& b : bt->items)
     if (b.phase == 0) fresh.push_back(&b);
   start_members(bt, L, fresh);
@@ -1380,13 +1390,17 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   if (active.empty()) return 0;
   clear_escalated(L, active);
   for (BItem* b : active) {
-    if (!b->gauss || b->iter == 0 || b->refac) (b->esc ? fac32 : fac).push_back(b);
+    if (!b->gauss || b->iter == 0 || b->refac)
+      (b->esc ? fac32 : fac).push_back(b);
+    else
+      nofac.push_back(b);
     b->refac = false;
   }
   const Batch ba = make_batch(L, 1, active);
   const Batch bf = make_batch(L, 2, fac);
   const Batch bf32 = make_batch(L, 6, fac32);
-  newton_iter(ba, &bf, &bf32, prec, bt->refine, L.gather, s);
+  const Batch bn = make_batch(L, 8, nofac);
+  newton_iter(ba, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
   int rc = check_launch();
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(L.hstep, L.gather, active.size() * 8, hipMemcpyDeviceToHost, s));

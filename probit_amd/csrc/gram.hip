@@ -156,8 +156,13 @@ void launch_gram_self(const Batch& b, hipStream_t s) {
 }
 
 // ---- B = I + S K S (Laplace) or sigma^2 I + K (VB) into the factor storage ----
+// With rs >= 0 the same pass over K also yields the per-tile partial sums of
+// the symmetric product K x, x = vector slot rs, in the slots k_symv_reduce
+// adds (row sums of tile (I, J) in part[I][J], column sums in part[J][I];
+// see k_symv_tiles), so the Newton step's K r costs no second read of K.
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, Sched sc, int vb) {
+__global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, Sched sc, int vb,
+                                                     int rs) {
   int bx;
   const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int Tn = P.Np / TILE;
@@ -172,9 +177,11 @@ __global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, Sched s
   const double* S = vslot(P, V_S);
   const double si = S[i];
   const double s2 = P.noise_std * P.noise_std;
+  double k[16];
   for (int c = 0; c < 16; ++c) {
     const int j = tj * TILE + jg * 16 + c;
     const double kij = P.K[(size_t)i + (size_t)j * ld];
+    k[c] = kij;
     double v;
     if (vb) {
       v = kij + ((i == j) ? ((i < P.N) ? s2 : 1.0) : 0.0);
@@ -183,9 +190,43 @@ __global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, Sched s
     }
     L[(size_t)i + (size_t)j * ld] = (T)v;
   }
+  if (rs < 0) return;
+  __shared__ double rpart[4][TILE];
+  const double* x = vslot(P, rs);
+  double rsum = 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) rsum += k[c] * x[tj * TILE + jg * 16 + c];
+  rpart[jg][il] = rsum;
+  double* part = P.symv;
+  if (ti != tj) {
+    // column sums of the wave's 16 columns by the transposing butterfly of
+    // k_symv_tiles: lane l ends with column jg * 16 + ((l >> 2) & 15)
+    const double xi = x[i];
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = k[q] * xi;
+#pragma unroll
+    for (int h = 8, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+      const bool up = (il & bit) != 0;
+#pragma unroll
+      for (int q = 0; q < h; ++q) {
+        const double give = up ? v[q] : v[q + h];
+        const double keep = up ? v[q + h] : v[q];
+        v[q] = keep + __shfl_xor(give, bit);
+      }
+    }
+    double cs = v[0];
+    cs += __shfl_xor(cs, 2);
+    cs += __shfl_xor(cs, 1);
+    if ((il & 3) == 0) part[((size_t)tj * Tn + ti) * TILE + jg * 16 + ((il >> 2) & 15)] = cs;
+  }
+  __syncthreads();
+  if (jg == 0)
+    part[((size_t)ti * Tn + tj) * TILE + il] =
+        (rpart[0][il] + rpart[1][il]) + (rpart[2][il] + rpart[3][il]);
 }
 
-void launch_form_B(const Batch& b, int vb, hipStream_t s) {
+void launch_form_B(const Batch& b, int vb, hipStream_t s, int rs) {
   // all members of a batch share one storage precision (prec in vb's high bits)
   const int prec = vb >> 8;
   const int isvb = vb & 0xff;
@@ -193,9 +234,9 @@ void launch_form_B(const Batch& b, int vb, hipStream_t s) {
       b, [&](int m) { return lower_tiles(b.np(m)); },
       [&](Prob* const* dp, const Sched& sc, int n) {
         if (prec == PGP_PREC_FP64)
-          hipLaunchKernelGGL(k_form_B<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, isvb);
+          hipLaunchKernelGGL(k_form_B<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, isvb, rs);
         else
-          hipLaunchKernelGGL(k_form_B<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, isvb);
+          hipLaunchKernelGGL(k_form_B<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc, isvb, rs);
       });
 }
 

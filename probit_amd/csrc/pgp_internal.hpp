@@ -232,7 +232,9 @@ bool timer_on(int cls);
 
 // gram.hip
 void launch_gram_self(const Batch& b, hipStream_t s);
-void launch_form_B(const Batch& b, int prec_vb, hipStream_t s);  // (prec << 8) | vb
+// (prec << 8) | vb; rs >= 0: also the partial sums of K x_rs (see
+// launch_symv_reduce)
+void launch_form_B(const Batch& b, int prec_vb, hipStream_t s, int rs = -1);
 void launch_cross_prep(const Batch& b, int prec, int maxM, hipStream_t s);
 int launch_gram_raw(const pgp_kernel_t* k, int dtype, const double* X, int64_t N,
                     const double* Z, int64_t M, void* out, int64_t ld, hipStream_t s);
@@ -277,6 +279,9 @@ enum VecOp {
 };
 enum RedWhat { RED_STEP = 0, RED_FINAL = 1 };
 void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s);
+// the two halves of launch_gemv_sym's symmetric product (modes < 10)
+void launch_symv_tiles(const Batch& b, int x_slot, hipStream_t s);
+void launch_symv_reduce(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s);
 void launch_vec_op(const Batch& b, int op, hipStream_t s);
 void launch_copy_slot(const Batch& b, int src, int dst, hipStream_t s);
 void launch_reduce(const Batch& b, int what, double* gather, hipStream_t s);

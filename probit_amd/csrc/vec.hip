@@ -167,6 +167,11 @@ void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStre
         });
     return;
   }
+  launch_symv_tiles(b, x_slot, s);
+  launch_symv_reduce(b, x_slot, out_slot, mode, s);
+}
+
+void launch_symv_tiles(const Batch& b, int x_slot, hipStream_t s) {
   sched_launch(
       b,
       [&](int m) {
@@ -176,6 +181,9 @@ void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStre
       [&](Prob* const* dp, const Sched& sc, int n) {
         hipLaunchKernelGGL(k_symv_tiles, dim3(n), dim3(NTHREADS), 0, s, dp, sc, x_slot);
       });
+}
+
+void launch_symv_reduce(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s) {
   sched_launch(
       b, [&](int m) { return (b.np(m) + NTHREADS - 1) / NTHREADS; },
       [&](Prob* const* dp, const Sched& sc, int n) {
