@@ -267,6 +267,7 @@ __global__ __launch_bounds__(NTHREADS) void k_cross_prep(Prob* const* probs, Sch
   const double si = vslot(P, V_S)[i];
   T* S = (T*)P.S;
   const double* xi = &xs[il * D];
+  double pv[16];
   for (int c = 0; c < 16; ++c) {
     const int jl = jg * 16 + c;
     const int j = tj * TILE + jl;
@@ -274,11 +275,25 @@ __global__ __launch_bounds__(NTHREADS) void k_cross_prep(Prob* const* probs, Sch
     if (i < P.N && j < M)
       v = kernel_value<double>(P.kern_kind, P.periodic, D, P.variance, ks, xi, &zs[jl * D]);
     S[(size_t)i + (size_t)j * P.Np] = (T)(si * v);
-    // wave reduction of v * a over the 64 rows of this tile (fixed order)
-    double pa = v * a;
-    for (int off = 32; off > 0; off >>= 1) pa += __shfl_xor(pa, off);
-    if (il == 0) part[jg][c] = pa;
+    pv[c] = v * a;
   }
+  // sums of v * a over the 64 rows of the tile for the wave's 16 columns, by
+  // a transposing butterfly (fixed order): lane l ends with column
+  // jg * 16 + ((l >> 2) & 15)
+#pragma unroll
+  for (int h = 8, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+    const bool up = (il & bit) != 0;
+#pragma unroll
+    for (int q = 0; q < h; ++q) {
+      const double give = up ? pv[q] : pv[q + h];
+      const double keep = up ? pv[q + h] : pv[q];
+      pv[q] = keep + __shfl_xor(give, bit);
+    }
+  }
+  double pa = pv[0];
+  pa += __shfl_xor(pa, 2);
+  pa += __shfl_xor(pa, 1);
+  if ((il & 3) == 0) part[jg][(il >> 2) & 15] = pa;
   __syncthreads();
   if (threadIdx.x < TILE) {
     const int jg2 = threadIdx.x >> 4, c2 = threadIdx.x & 15;
