@@ -128,7 +128,7 @@ int pgp_version(void);
 
 /* Opt-in launch timer for measurement: with bit c of `mask` set, launches of
  * kernel class c (0 = outer trailing SYRK update of the Cholesky, 1 = its
- * per-column steps: diagonal factor, panel, inner update, 2 = TRSV,
+ * per-column steps: diagonal factor, panel, column update, 2 = TRSV,
  * 3 = Gram assembly, 4 = TRSM) are bracketed by HIP events on their stream,
  * with the algorithmic flops / bytes each launch performs.  pgp_timer_read
  * waits for the recorded events and returns the class totals.  Not

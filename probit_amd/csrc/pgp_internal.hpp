@@ -221,7 +221,7 @@ __device__ __forceinline__ int wait_progress(uint64_t* progress, uint32_t epoch,
 // Opt-in launch timer: HIP events around the launches of one kernel class
 // on the launching stream, with the algorithmic flops each launch performs.
 //   TC_SYRK  outer (rank 64 G) trailing update of the Cholesky
-//   TC_PANEL per-column Cholesky steps: diagonal factor, panel, inner update
+//   TC_PANEL per-column Cholesky steps: diagonal factor, panel, column update
 //   TC_TRSV  single-launch triangular solves
 //   TC_GRAM  Gram assembly
 //   TC_TRSM  multi-right-hand-side solves
