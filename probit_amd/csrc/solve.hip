@@ -493,11 +493,34 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm_rows(Prob* const* probs, Sche
   const Prob& P = *probs[m];
   if (ct >= P.Mp / TILE || i >= P.Np / TILE) return;
   uint64_t* prog = P.progress_ct + ct;
-  const T* L = (const T*)P.L;
+  const auto L = gbl((const T*)P.L);
   T* S = (T*)P.S;
   const int64_t ld = P.Np;
   const int64_t ri = (int64_t)i * TILE, n0 = (int64_t)ct * TILE;
   const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
+  // Operands that exist before the launch are fetched into registers ahead
+  // of the waits: DINV_i and S_i at once, the next L_ij while X_j is awaited.
+  T dv[16], la[16], sv[2][2][4];
+  {
+    const auto D = gbl(P.dinv + (size_t)i * TILE * TILE);
+    const auto Sg = gbl((const T*)P.S);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dv[q] = (T)D[r + (size_t)(cb + 4 * q) * TILE];
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          sv[bm][bn][q] =
+              Sg[(size_t)(ri + acc_row<T>(bm, q)) + (size_t)(n0 + acc_col(bn)) * ld];
+  }
+  auto fetch_l = [&](int j) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      la[q] = L[(size_t)(ri + r) + (size_t)((int64_t)j * TILE + cb + 4 * q) * ld];
+  };
+  if (i > 0) fetch_l(0);
   acc_t acc[2][2];
   zero_acc<T>(acc);
   int done = 0, ready = 0;
@@ -511,7 +534,8 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm_rows(Prob* const* probs, Sche
     for (; done < ready; ++done) {
       const int64_t rj = (int64_t)done * TILE;
       __syncthreads();  // the previous product is done with LDS
-      load_block_kmajor<T, T>(As, PP, L, ld, ri, rj);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) As[(cb + 4 * q) * PP + r] = la[q];
       // B[kk][n] = X_j(kk, n), published in this launch: coherent loads
 #pragma unroll 4
       for (int q = 0; q < 16; ++q) {
@@ -519,6 +543,7 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm_rows(Prob* const* probs, Sche
         Bs[r * PB + c] = __hip_atomic_load(&S[(size_t)(rj + r) + (size_t)(n0 + c) * ld],
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      if (done + 1 < i) fetch_l(done + 1);
       __syncthreads();
       mma64_kmajor<T>(As, PP, Bs, PB, acc);
     }
@@ -532,9 +557,10 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm_rows(Prob* const* probs, Sche
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = acc_row<T>(bm, q), col = acc_col(bn);
-        Bs[row * PB + col] = S[(size_t)(ri + row) + (size_t)(n0 + col) * ld] - acc[bm][bn][q];
+        Bs[row * PB + col] = sv[bm][bn][q] - acc[bm][bn][q];
       }
-  load_block_kmajor<double, T>(As, PP, P.dinv + (size_t)i * TILE * TILE, TILE, 0, 0);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) As[(cb + 4 * q) * PP + r] = dv[q];
   __syncthreads();
   zero_acc<T>(acc);
   mma64_kmajor<T>(As, PP, Bs, PB, acc);
