@@ -353,7 +353,7 @@ __device__ __forceinline__ int xcd_contiguous(int g, int total) {
 // FUSE: the workgroup given factor = true (the tile that becomes the next
 // diagonal block) keeps its updated tile in LDS and factors it at once.
 template <typename T, int BF16, int FUSE>
-__device__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj,
+__device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj,
                                 bool factor, const FactorOut& o) {
   auto A = gbl(A_);
   constexpr int PITCH = BF16 ? 72 : KP<T>::value;
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
 // rounded to T), then held in LDS as FP64: out[r][c] (kmajor = false) or
 // out[c][r] (kmajor = true).  `arena` holds the operands first.
 template <typename T, int BF16, typename OT>
-__device__ void update_tile_to_lds(const T* A_, int64_t ld, int kc0, int kc1, int64_t ri,
+__device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int kc0, int kc1, int64_t ri,
                                    int64_t rj, char* arena, OT* out, int opitch, bool kmajor) {
   auto A = gbl(A_);
   constexpr int PITCH = BF16 ? 72 : KP<T>::value;

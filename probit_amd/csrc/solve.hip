@@ -382,7 +382,7 @@ int launch_trsv_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const d
 // lower_rhs: S is lower triangular (identity-like right-hand side): tiles
 // with ct > k, and the updates of tile ct == k, are zero and are skipped.
 template <typename T>
-__device__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* S, int64_t lds,
+__device__ __forceinline__ void trsm_step(const T* L, int64_t ld, const double* dinv, int k, T* S, int64_t lds,
                           int w, int ct, int lower_rhs) {
   constexpr int PP = KP<T>::value;  // A operand: conflict-free MFMA reads
   constexpr int PB = TILE + 1;       // B operand: written transposed, conflict-free stores
@@ -608,7 +608,7 @@ constexpr int TRSM_G = 4;
 // launch: they are read with coherent loads (past the CU's L1, which may
 // still hold the values they replaced).
 template <typename T>
-__device__ void trsm_acc(const T* L, int64_t ld, const T* S, int64_t lds, int64_t r0, int64_t n0,
+__device__ __forceinline__ void trsm_acc(const T* L, int64_t ld, const T* S, int64_t lds, int64_t r0, int64_t n0,
                          int c0, int c1, int c_fresh, T* As, T* Bs,
                          typename Acc<T>::type (&acc)[2][2]) {
   constexpr int PP = KP<T>::value;  // A operand: conflict-free MFMA reads
@@ -646,7 +646,7 @@ __device__ void trsm_acc(const T* L, int64_t ld, const T* S, int64_t lds, int64_
 // Workgroup `u` of launch q for one right-hand side of nct column tiles:
 // u < nct solves SS_q for column tile u; the others update one row block.
 template <typename T>
-__device__ void trsm_ss(const T* L, int64_t ld, const double* dinv, int nb, int q, T* S,
+__device__ __forceinline__ void trsm_ss(const T* L, int64_t ld, const double* dinv, int nb, int q, T* S,
                         int64_t lds, int nct, int u, int lower_rhs) {
   constexpr int PP = KP<T>::value;
   constexpr int PB = TILE + 1;

@@ -223,14 +223,15 @@ def test_serving_alone_equals_in_mix():
     assert np.array_equal(a[2].var.cpu().numpy(), b[2].var.cpu().numpy())
 
 
-@pytest.mark.parametrize("mode", ["fp64", "fp32"])
-def test_serving_schedule_independent(mode):
+@pytest.mark.parametrize("mode,n", [("fp64", 1024), ("fp32", 1024), ("fp32", 1088)])
+def test_serving_schedule_independent(mode, n):
     """A member's results do not depend on the factorisation schedule its
     batch gets: alone, the N = 1024 request is factored by the one-launch-per-
     column kernel throughout; next to fifteen N = 2560 requests its first
-    super-steps go through the separate update / factor / panel kernels."""
+    super-steps go through the separate update / factor / panel kernels
+    (N = 1088: an odd number of block columns, a partial last super-step)."""
     from probit_amd.serving import ServingEngine
-    target = small_problem(90, 1024, 2, "probit", "EQ")
+    target = small_problem(90, n, 2, "probit", "EQ")
     Xt = np.random.default_rng(91).normal(size=(64, 2))
     others = [small_problem(100 + i, 2560, 2, "gaussian", "Matern12") for i in range(15)]
 
