@@ -88,19 +88,20 @@ def serve_window(engine, window, t_arrive=None, chunk=128):
     res = {}
     while pending or engine.in_flight:
         launched = engine.launch()
-        still, n = [], 0
-        for r in pending:
-            if n >= chunk or (t_arrive is not None and t_arrive[r["rid"]] > _t.perf_counter()):
-                still.append(r)
-                continue
+        # requests are offered in order; the first one refused (no slot or no
+        # HBM) ends this round of submissions
+        n = 0
+        while n < len(pending) and n < chunk:
+            r = pending[n]
+            if t_arrive is not None and t_arrive[r["rid"]] > _t.perf_counter():
+                break
             p = r["p"]
-            if engine.submit(r["rid"], r["X"], r["y"], p.kind, p.lengthscale, p.variance, p.lik,
-                             p.noise_std, p.cutpoints, r["Xt"], p.period):
-                t_sub[r["rid"]] = _t.perf_counter()
-                n += 1
-            else:
-                still.append(r)
-        pending = still
+            if not engine.submit(r["rid"], r["X"], r["y"], p.kind, p.lengthscale, p.variance,
+                                 p.lik, p.noise_std, p.cutpoints, r["Xt"], p.period):
+                break
+            t_sub[r["rid"]] = _t.perf_counter()
+            n += 1
+        pending = pending[n:]
         if not launched:
             continue
         for c in engine.finish():

@@ -69,11 +69,14 @@ class ServingEngine:
         self._launched = False
 
     # -- sizing ---------------------------------------------------------------
+    def _sizes(self, N: int, D: int, M: int):
+        """(workspace bytes, predict workspace bytes, request bytes)."""
+        nb = int(self.lib.pgp_workspace_size(N, D, self.prec))
+        pb = int(self.lib.pgp_predict_workspace_size(N, M, self.prec)) if M > 0 else 0
+        return nb, pb, nb + pb + 8 * (N * (D + 3) + M * (D + 2))
+
     def request_bytes(self, N: int, D: int, M: int) -> int:
-        b = self.lib.pgp_workspace_size(N, D, self.prec)
-        if M > 0:
-            b += self.lib.pgp_predict_workspace_size(N, M, self.prec)
-        return int(b + 8 * (N * (D + 3) + M * (D + 2)))
+        return self._sizes(N, D, M)[2]
 
     def can_admit(self, N: int, D: int, M: int) -> bool:
         return (len(self._slots) < self.capacity
@@ -92,14 +95,15 @@ class ServingEngine:
         """Queue one posterior; returns False (nothing queued) if it does not fit now.
         Device tensors are used in place; host arrays are copied to the GPU.
         One device block per request holds its workspaces and outputs."""
+        if len(self._slots) >= self.capacity:  # no slot: nothing else to check
+            return False
         N, D = int(X.shape[0]), int(X.shape[1])
         M = 0 if Xtest is None else int(Xtest.shape[0])
-        if not self.can_admit(N, D, M):
+        nb, pb, nbytes = self._sizes(N, D, M)
+        if self.used + nbytes > self.budget:
             return False
         if D > _lib.MAX_DIM:
             raise ValueError(f"input dimension must be in [1, {_lib.MAX_DIM}] (got {D})")
-        nb = self.lib.pgp_workspace_size(N, D, self.prec)
-        pb = self.lib.pgp_predict_workspace_size(N, M, self.prec) if M > 0 else 0
         vec = (8 * N + 255) // 256 * 256
         mv = (8 * M + 255) // 256 * 256
         # [workspace | predict workspace | weight | precision | mean | var]
@@ -151,7 +155,7 @@ class ServingEngine:
         _lib.check(rc, "pgp_batcher_submit")
         s = _Slot()
         s.item, s.tensors = item, (keep, N, M, o_w, o_p, o_m, o_v)
-        s.nbytes = self.request_bytes(N, D, M)
+        s.nbytes = nbytes
         s.t_submit = time.perf_counter()
         self._slots[int(tag)] = s
         self.used += s.nbytes
