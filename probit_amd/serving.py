@@ -94,7 +94,10 @@ class ServingEngine:
                cutpoints=None, Xtest=None, period=None) -> bool:
         """Queue one posterior; returns False (nothing queued) if it does not fit now.
         Device tensors are used in place; host arrays are copied to the GPU.
-        One device block per request holds its workspaces and outputs."""
+        Per request, one device block holds its workspaces (released when the
+        request completes, so the allocator's cached blocks serve the next
+        requests) and a small one its outputs (what the completion's tensors
+        view)."""
         if len(self._slots) >= self.capacity:  # no slot: nothing else to check
             return False
         N, D = int(X.shape[0]), int(X.shape[1])
@@ -106,14 +109,16 @@ class ServingEngine:
             raise ValueError(f"input dimension must be in [1, {_lib.MAX_DIM}] (got {D})")
         vec = (8 * N + 255) // 256 * 256
         mv = (8 * M + 255) // 256 * 256
-        # [workspace | predict workspace | weight | precision | mean | var]
+        # workspaces [workspace | predict workspace]; outputs [weight |
+        # precision | mean | var]
         o_pw = (nb + 255) // 256 * 256
-        o_w = o_pw + (pb + 255) // 256 * 256
+        ws = torch.empty(o_pw + (pb + 255) // 256 * 256, dtype=torch.uint8, device=self.device)
+        o_w = 0
         o_p = o_w + vec
         o_m = o_p + vec
         o_v = o_m + mv
-        block = torch.empty(o_v + mv, dtype=torch.uint8, device=self.device)
-        base = block.data_ptr()
+        outb = torch.empty(o_v + mv, dtype=torch.uint8, device=self.device)
+        wbase, base = ws.data_ptr(), outb.data_ptr()
         Xd, yd = self._dev(X), self._dev(y)
         item = _lib.BatchItemT()
         pr = item.problem
@@ -140,16 +145,16 @@ class ServingEngine:
             lk.kind = _lib.LIK_PROBIT
             lk.J = cut.shape[0] - 1
             lk.cutpoints[:cut.shape[0]] = cut.tolist()
-        item.workspace = base
+        item.workspace = wbase
         item.workspace_bytes = nb
-        keep = [block, Xd, yd]
+        keep = [outb, ws, Xd, yd]
         if M > 0:
             Xt = self._dev(Xtest)
             keep.append(Xt)
             item.Xtest = Xt.data_ptr()
             item.M = M
             item.mean, item.var = base + o_m, base + o_v
-            item.predict_workspace = base + o_pw
+            item.predict_workspace = wbase + o_pw
             item.predict_workspace_bytes = pb
         rc = self.lib.pgp_batcher_submit(self._h, item, int(tag))
         _lib.check(rc, "pgp_batcher_submit")
