@@ -208,13 +208,18 @@ class ServingEngine:
         return self._launched
 
     def finish(self) -> list[Completed]:
-        """Wait until a lane's step is done; returns the requests it completed."""
+        """Wait until a lane's step is done, start the next step of every idle
+        lane at once (before the completions are collected on the host, so
+        the lane does not wait for that), and return the requests it
+        completed."""
         if not self._launched:
             return []
         n = C.c_int32(0)
         rc = self.lib.pgp_batcher_finish(self._h, self._done_buf, self.capacity, C.byref(n))
-        self._launched = self.lib.pgp_batcher_launched(self._h) > 0
         _lib.check(rc, "pgp_batcher_finish")
+        if len(self._slots) > n.value:  # requests still in flight or waiting
+            _lib.check(self.lib.pgp_batcher_launch(self._h), "pgp_batcher_launch")
+        self._launched = self.lib.pgp_batcher_launched(self._h) > 0
         return self._collect(n.value)
 
     @property
