@@ -89,9 +89,11 @@ def serve_window(engine, window, t_arrive=None, chunk=128):
     while pending or engine.in_flight:
         launched = engine.launch()
         # requests are offered in order; the first one refused (no slot or no
-        # HBM) ends this round of submissions
+        # HBM), or a lane's step being done, ends this round of submissions
         n = 0
         while n < len(pending) and n < chunk:
+            if n % 8 == 7 and engine.ready():
+                break
             r = pending[n]
             if t_arrive is not None and t_arrive[r["rid"]] > _t.perf_counter():
                 break

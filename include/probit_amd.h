@@ -321,13 +321,16 @@ int pgp_batcher_step(pgp_batcher_t* batcher, int64_t* done_tags,
  * its previous step, one Newton iteration for its other items), returning
  * at once; finish() waits until at least one lane's step is done and returns
  * the tags of the items it completed; launched() counts the lanes with a
- * step in flight.  An item is reported one step after it converges; its
+ * step in flight; ready() counts the lanes whose step is done (finish()
+ * would return at once) without waiting.  An item is reported one step
+ * after it converges; its
  * results are the same as with pgp_batcher_step.  Do not call
  * pgp_batcher_step while a lane has a step in flight. */
 int pgp_batcher_launch(pgp_batcher_t* batcher);
 int pgp_batcher_finish(pgp_batcher_t* batcher, int64_t* done_tags,
                        int32_t max_done, int32_t* n_done);
 int32_t pgp_batcher_launched(pgp_batcher_t* batcher);
+int32_t pgp_batcher_ready(pgp_batcher_t* batcher);
 int32_t pgp_batcher_in_flight(pgp_batcher_t* batcher);
 int pgp_batcher_destroy(pgp_batcher_t* batcher);
 

@@ -159,6 +159,7 @@ _SIGNATURES = {
     "pgp_batcher_finish": (C.c_int, [_P, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int32)]),
     "pgp_batcher_in_flight": (_I32, [_P]),
     "pgp_batcher_launched": (_I32, [_P]),
+    "pgp_batcher_ready": (_I32, [_P]),
     "pgp_batcher_destroy": (C.c_int, [_P]),
 }
 

@@ -1453,6 +1453,17 @@ int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
   }
 }
 
+int32_t pgp_batcher_ready(pgp_batcher_t* bt) {
+  if (!bt) return 0;
+  int32_t n = 0;
+  for (int li = 0; li < NLANES; ++li) {
+    const Lane& L = bt->lane[li];
+    // a failed step counts as ready: pgp_batcher_finish reports it
+    if (L.launched && hipEventQuery(L.ev) != hipErrorNotReady) ++n;
+  }
+  return n;
+}
+
 int pgp_batcher_destroy(pgp_batcher_t* bt) {
   if (!bt) return 0;
   for (int li = 0; li < NLANES; ++li) {

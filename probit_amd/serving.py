@@ -222,6 +222,10 @@ class ServingEngine:
         self._launched = self.lib.pgp_batcher_launched(self._h) > 0
         return self._collect(n.value)
 
+    def ready(self) -> bool:
+        """True if a lane's step is done (finish() would return at once)."""
+        return self._launched and self.lib.pgp_batcher_ready(self._h) > 0
+
     @property
     def in_flight(self) -> int:
         return len(self._slots)
