@@ -50,7 +50,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--requests-per-step", type=int, default=512)
     p.add_argument("--precision", default="fp32", choices=["bf16", "fp32", "fp64"])
-    p.add_argument("--capacity", type=int, default=512)
+    p.add_argument("--capacity", type=int, default=1024,
+                   help="request slots of the serving engine (requests in flight)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-sized device buffer 3 times and exit (the known-size "
                         "reference of the HBM traffic counters)")
