@@ -175,6 +175,21 @@ int pgp_probit_predictive(const pgp_likelihood_t* lik, const double* mean,
 int pgp_potrf(int precision, void* A, int64_t N, int64_t ld, double* dinv,
               double* logdiag, int32_t* info, void* stream);
 
+/* GP prior samples f = L_K z, L_K the lower Cholesky factor of
+ * K + jitter I with K = kernel(X, X) (FP64 throughout): z and f are N x S,
+ * column-major (one sample per column, leading dimensions ldz, ldf >= N).
+ * The workspace (pgp_prior_sample_workspace_size(N) bytes, caller-owned)
+ * receives K and its factor (padded to a multiple of 64 with the identity).
+ * A K + jitter I that is not positive definite gives NaN samples, as the
+ * reference's jnp.linalg.cholesky does.  Replaces the sampling in the data
+ * generators (examples/regression.py:52-58,
+ * examples/classification.py:227-234). */
+int64_t pgp_prior_sample_workspace_size(int64_t N);
+int pgp_prior_sample(const pgp_kernel_t* kernel, const double* X, int64_t N,
+                     double jitter, const double* z, int64_t S, int64_t ldz,
+                     double* f, int64_t ldf, void* workspace,
+                     int64_t workspace_bytes, void* stream);
+
 /* Triangular solve with the lower factor: trans = 0 solves L x = b,
  * trans = 1 solves L^T x = b.  dinv (caller-owned, N * 64 doubles) receives
  * the FP64 inverses of the 64 x 64 diagonal blocks; b (N doubles) is

@@ -565,3 +565,15 @@ def probit_predictive_distributions(lik, mean, variance):
         hi = 1.0 if cut[j + 1] == np.inf else ndtr((cut[j + 1] - mean) / std)
         out[:, j] = hi - lo
     return out
+
+
+# ----------------------------------------------------------------------------
+# GP prior samples (examples/regression.py:52-58, examples/classification.py:227-234)
+# ----------------------------------------------------------------------------
+def prior_sample(spec: KernelSpec, X: np.ndarray, z: np.ndarray, jitter: float = 1e-10):
+    """f = L_K z with L_K = chol(K + jitter I), K = kernel(X) (the generators'
+    ``K = B.dense(kernel(X)); K = K + jitter * I; L_K = cholesky(K); f = L_K @ z``).
+    z is (N,) or (N, S); f has its shape."""
+    K = gram(spec, X) + jitter * np.eye(X.shape[0])
+    L = np.linalg.cholesky(K)
+    return L @ z

@@ -132,6 +132,9 @@ _SIGNATURES = {
     "pgp_likelihood_eval": (C.c_int, [C.POINTER(LikelihoodT), _P, _P, _I64, _P, _P, _P, _P]),
     "pgp_probit_predictive": (C.c_int, [C.POINTER(LikelihoodT), _P, _P, _I64, _P, _P]),
     "pgp_potrf": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _P, _P]),
+    "pgp_prior_sample_workspace_size": (_I64, [_I64]),
+    "pgp_prior_sample": (C.c_int, [C.POINTER(KernelT), _P, _I64, C.c_double, _P, _I64, _I64, _P,
+                                   _I64, _P, _I64, _P]),
     "pgp_trsv": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _P, C.c_int, _P]),
     "pgp_trsm": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _I64, _I64, _P]),
     "pgp_workspace_size": (_I64, [_I64, _I32, _I32]),

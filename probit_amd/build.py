@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libprobit_amd.so"
 BUILD_DIR = HERE.parent / "_build"
 
-SOURCES = ["gram.hip", "lik.hip", "chol.hip", "solve.hip", "vec.hip", "grad.hip",
+SOURCES = ["gram.hip", "lik.hip", "chol.hip", "solve.hip", "vec.hip", "grad.hip", "sample.hip",
            "api.hip"]
 
 HIPCC_FLAGS = [

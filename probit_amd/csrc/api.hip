@@ -518,6 +518,28 @@ int pgp_potrf(int precision, void* A, int64_t N, int64_t ld, double* dinv, doubl
   return check_launch();
 }
 
+int64_t pgp_prior_sample_workspace_size(int64_t N) {
+  if (N <= 0) return -1;
+  return prior_ws_bytes((N + TILE - 1) / TILE * TILE);
+}
+
+int pgp_prior_sample(const pgp_kernel_t* kernel, const double* X, int64_t N, double jitter,
+                     const double* z, int64_t S, int64_t ldz, double* f, int64_t ldf,
+                     void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!kernel) return fail(-1, "kernel is NULL");
+  if (kernel->dim <= 0 || kernel->dim > PGP_MAX_DIM) return fail(-1, "kernel dim out of range");
+  if (!X || N <= 0) return fail(-2, "X / N invalid");
+  if (!(jitter >= 0.0)) return fail(-4, "jitter must be >= 0");
+  if (!z || S <= 0 || ldz < N) return fail(-5, "z / S / ldz invalid");
+  if (!f || ldf < N) return fail(-8, "f / ldf invalid");
+  if (!workspace || workspace_bytes < pgp_prior_sample_workspace_size(N))
+    return fail(-10, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+                (long long)pgp_prior_sample_workspace_size(N));
+  launch_prior_sample(kernel, X, N, jitter, z, S, ldz, f, ldf, (char*)workspace,
+                      (hipStream_t)stream);
+  return check_launch();
+}
+
 int pgp_trsv(int precision, const void* L, int64_t N, int64_t ld, double* dinv, double* b,
              double* x, int trans, void* stream) {
   if (precision < 0 || precision > 2) return fail(-1, "unknown precision %d", precision);

@@ -253,6 +253,17 @@ void launch_potrf(const Batch& b, int prec, hipStream_t s);
 void launch_dinv(const Batch& b, int prec, hipStream_t s);
 int launch_potrf_raw(int prec, void* A, int64_t N, int64_t ld, double* dinv, double* logdiag,
                      int32_t* info, hipStream_t s);
+
+// Prior sampling workspace (padded size Np): K / its factor (Np x Np FP64),
+// the diagonal-block inverses (Np x 64 FP64), the factor status word.
+inline int64_t prior_ws_offset_dinv(int64_t Np) { return (Np * Np * 8 + 255) / 256 * 256; }
+inline int64_t prior_ws_offset_info(int64_t Np) {
+  return prior_ws_offset_dinv(Np) + (Np * TILE * 8 + 255) / 256 * 256;
+}
+inline int64_t prior_ws_bytes(int64_t Np) { return prior_ws_offset_info(Np) + 256; }
+int launch_prior_sample(const pgp_kernel_t* kernel, const double* X, int64_t N, double jitter,
+                        const double* z, int64_t S, int64_t ldz, double* f, int64_t ldf,
+                        char* ws, hipStream_t s);
 int launch_dinv_raw(int prec, const void* A, int64_t N, int64_t ld, double* dinv, hipStream_t s);
 
 // solve.hip

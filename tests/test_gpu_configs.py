@@ -217,3 +217,32 @@ def test_c4_lbfgsb_50_steps():
     assert np.all(np.isfinite(hist[-5:]))
     cut_opt = opt[1][1]
     assert np.all(np.diff(cut_opt) > 0)
+
+
+def test_prior_sample_c2_latent_n12288():
+    """The c2 latent drawn exactly on the GPU: N = 8192 train + 4096 test
+    points, X ~ N(0, I_2), Matern12 l = 1, jitter 1e-6 (FP64 Gram, factor and
+    product), against the oracle's FP64 chol(K + jitter I) z on the host."""
+    import torch
+
+    import probit_amd as P
+    from probit_amd.prior import sample_prior
+    rng = np.random.default_rng(1)
+    N = 12288
+    X = rng.normal(size=(N, 2))
+    z = rng.standard_normal(N)
+    kern = P.Matern12().stretch(1.0)
+    Xd = torch.as_tensor(X, device="cuda")
+    zd = torch.as_tensor(z, device="cuda")
+    sample_prior(kern, Xd, zd, jitter=1e-6)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f = sample_prior(kern, Xd, zd, jitter=1e-6)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    ref = O.prior_sample(O.KernelSpec.make(O.MATERN12, 1.0, 1.0, 2), X, z, 1e-6)
+    err = _rel(f.cpu().numpy(), ref)
+    flops = N**3 / 3 + float(N) * N  # factor + triangular product
+    print(json.dumps({"config": "f4-prior-sample", "N": N, "seconds": round(secs, 4),
+                      "fp64_tflops": round(flops / secs / 1e12, 2), "rel_err_vs_fp64_host": err}))
+    assert err < 1e-8

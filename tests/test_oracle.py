@@ -343,3 +343,33 @@ def test_predictive_distributions_sum_to_one():
     assert np.allclose(P.sum(axis=1), 1.0)
     s = np.sqrt(var + 0.25)
     assert np.allclose(P[:, 0], stats.norm.cdf((-1.0 - mean) / s))
+
+
+def test_prior_sample_factor_and_covariance():
+    """The oracle's prior sample is L z with L L^T = K + jitter I: the identity
+    as z returns L itself (lower triangular, reconstructing K + jitter I), and
+    the sample covariance of many draws is K + jitter I -- which L^T z (a
+    transposed factor) or a dropped jitter would miss."""
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, size=(6, 2))
+    spec = O.KernelSpec.make(O.EQ, 1.7, [0.6, 1.1], 2)
+    jitter = 0.3
+    K = O.gram(spec, X) + jitter * np.eye(6)
+    L = O.prior_sample(spec, X, np.eye(6), jitter)
+    assert np.allclose(np.triu(L, 1), 0.0)
+    assert np.allclose(L @ L.T, K, rtol=1e-12, atol=1e-12)
+    assert np.max(np.abs(L.T @ L - K)) > 0.1 * np.max(np.abs(K))  # the check below can tell
+    z = rng.standard_normal((6, 200000))
+    f = O.prior_sample(spec, X, z, jitter)
+    C = f @ f.T / z.shape[1]
+    assert np.max(np.abs(C - K)) < 0.03 * np.max(np.abs(K))
+
+
+def test_prior_sample_of_uncorrelated_points_scales_z():
+    """Points far apart under a short Matern12: K = variance I, so the sample
+    is sqrt(variance + jitter) z exactly."""
+    X = np.arange(5.0)[:, None] * 50.0
+    spec = O.KernelSpec.make(O.MATERN12, 2.0, 0.01, 1)
+    z = np.random.default_rng(6).standard_normal(5)
+    f = O.prior_sample(spec, X, z, 1e-3)
+    assert np.allclose(f, np.sqrt(2.0 + 1e-3) * z, rtol=1e-12, atol=0.0)
