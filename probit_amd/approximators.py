@@ -17,6 +17,7 @@ failures surface as NaN (as JAX would produce) plus ``last_result.status``.
 
 from __future__ import annotations
 
+import ctypes as C
 import warnings
 from abc import ABC, abstractmethod
 
@@ -316,7 +317,11 @@ class _LaplaceEvidence(torch.autograd.Function):
     @staticmethod
     def forward(ctx, gp, parameters, var, ls, per, noise, cut):
         ks, lsd = gp._structs(parameters)
-        res = gp._run_fit(ks, lsd)[2]
+        pre = getattr(gp, "_prefit", {}).get(gp._key(ks, lsd))
+        if pre is not None:  # fixed point from a batched fit: re-form its state
+            res = gp._set_state(ks, lsd, pre, None)[3]
+        else:
+            res = gp._run_fit(ks, lsd)[2]
         ctx.gp = gp
         ctx.key = gp._key(ks, lsd)
         ctx.structs = (ks, lsd)
@@ -362,6 +367,80 @@ class LaplaceGP(Approximator):
 
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
+        self._prefit = {}
+
+    def value_and_grad_batch(self, memory_fraction: float = 0.25):
+        """``vmap(value_and_grad(objective))`` over a list of parameter sets
+        (the lengthscale sweep of examples/classification.py:489-504).  The
+        Newton fits of all sets run together through the batcher (one launch
+        per blocked step for every set, the serving path's kernels); then, set
+        by set, B is re-formed and factored at its fixed point and the adjoint
+        gives the gradient.  Returns fn(list_of_parameters) -> (values (K,)
+        float64 NumPy, [grads shaped like each parameters]).  At most
+        ``memory_fraction`` of the free device memory holds fit workspaces at
+        once (larger sweeps run in chunks)."""
+        single = self.value_and_grad()
+
+        def fn(parameter_list):
+            plist = list(parameter_list)
+            structs = [self._structs(p) for p in plist]
+            weights = self._batched_fits(structs, memory_fraction)
+            values, grads = [], []
+            try:
+                for p, (ks, ls), w in zip(plist, structs, weights):
+                    self._prefit = {self._key(ks, ls): w}
+                    v, g = single(p)
+                    values.append(v)
+                    grads.append(g)
+            finally:
+                self._prefit = {}
+            return np.asarray(values, dtype=np.float64), grads
+
+        return fn
+
+    def _batched_fits(self, structs, memory_fraction):
+        """Weights at the fixed points of `structs` ((kernel, likelihood)
+        pairs) from the batcher, in order."""
+        lib = _lib.lib()
+        need = int(lib.pgp_workspace_size(self.N, self.D, self._prec))
+        free, _ = torch.cuda.mem_get_info(self.device)
+        chunk = max(1, min(len(structs), int(free * memory_fraction) // max(1, need), 512))
+        out = []
+        for c0 in range(0, len(structs), chunk):
+            part = structs[c0:c0 + chunk]
+            n = len(part)
+            tb = lib.pgp_batcher_table_size(n)
+            table = _lib.workspace(tb, self.device)
+            h = C.c_void_p()
+            _lib.check(lib.pgp_batcher_create(n, self._options(), table.data_ptr(), tb,
+                                              self._stream(), C.byref(h)), "pgp_batcher_create")
+            try:
+                items, keep = [], []
+                for i, (ks, ls) in enumerate(part):
+                    ws = _lib.workspace(need, self.device)
+                    w, prec = self._vec(), self._vec()
+                    item = _lib.BatchItemT()
+                    item.problem = self._problem(w, prec)
+                    item.kernel = ks
+                    item.lik = ls
+                    item.workspace = ws.data_ptr()
+                    item.workspace_bytes = ws.numel()
+                    _lib.check(lib.pgp_batcher_submit(h, item, i), "pgp_batcher_submit")
+                    items.append(item)
+                    keep.append((ws, w, prec))
+                done = (C.c_int64 * n)()
+                nd = C.c_int32(0)
+                finished = 0
+                while finished < n:
+                    _lib.check(lib.pgp_batcher_step(h, done, n, C.byref(nd)), "pgp_batcher_step")
+                    finished += nd.value
+                for it in items:
+                    if it.result.status & _lib.INFO_MAXITER:
+                        warnings.warn(f"Newton iteration stopped at maxiter={self.maxiter}")
+                out.extend(k[1] for k in keep)
+            finally:
+                lib.pgp_batcher_destroy(h)
+        return out
 
     def _run_fit(self, ks, ls):
         lib = _lib.lib()

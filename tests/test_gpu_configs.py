@@ -246,3 +246,39 @@ def test_prior_sample_c2_latent_n12288():
     print(json.dumps({"config": "f4-prior-sample", "N": N, "seconds": round(secs, 4),
                       "fp64_tflops": round(flops / secs / 1e12, 2), "rel_err_vs_fp64_host": err}))
     assert err < 1e-8
+
+
+def test_lengthscale_sweep_50_points_batched():
+    """The examples' lengthscale sweep (vmap(value_and_grad) over 50
+    lengthscales in logspace(-2, 2), examples/classification.py:489-504) on a
+    probit J=3 problem, N=1024: batched fits + per-set adjoint against the
+    one-set-at-a-time loop (same values)."""
+    import torch
+
+    import probit_amd as P
+    p = DG.evidence_gradient(seed=3, N=1024, lik="probit")
+    base = P.EQ if p.kind == DG.EQ else P.Matern12
+    gp = P.LaplaceGP(data=(p.X, p.y), prior=lambda th: float(p.variance) * base().stretch(th),
+                     log_likelihood=P.log_probit_likelihood, tolerance=1e-8)
+    lp = p.likelihood_parameters()
+    plist = [(float(x), lp) for x in np.logspace(-2, 2, 50)]
+    fb = gp.value_and_grad_batch()
+    fb(plist[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vals, grads = fb(plist)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    f1 = gp.value_and_grad()
+    t0 = time.perf_counter()
+    one = [f1(q) for q in plist]
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    fin = np.isfinite(vals)
+    dv = max(abs(v - o[0]) / max(1.0, abs(o[0])) for v, o, ok in zip(vals, one, fin) if ok)
+    print(json.dumps({"config": "f2-sweep", "N": 1024, "points": 50,
+                      "finite": int(fin.sum()), "seconds_batched": round(t_batch, 3),
+                      "seconds_one_at_a_time": round(t_loop, 3),
+                      "max_rel_value_diff": dv}))
+    assert fin.sum() >= 40
+    assert dv < 1e-9
