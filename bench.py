@@ -272,17 +272,20 @@ def main():
     ms_per_launch = ms.value / max(1, n_l.value)
     achieved = fl.value / (ms.value * 1e-3) / 1e12 if ms.value > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
-    # HBM bytes per launch of the outer update, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE runs of this workload scaled by the copy
-    # calibration (profiles/syrk_outer_traffic.json); null if not measured
+    # HBM bytes per launch of the outer update: the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated) give
+    # HBM bytes per algorithmic byte over their timed launches
+    # (profiles/syrk_outer_traffic.json); times this run's algorithmic bytes
+    # per launch, since the launch mix varies with the serving dynamics.
+    # null if not measured
+    algo_bytes = by.value / max(1, n_l.value)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "syrk_outer_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             rec = json.load(f).get(args.precision)
-        if rec:
-            traffic = rec.get("hbm_bytes_per_launch")
-    algo_bytes = by.value / max(1, n_l.value)
+        if rec and rec.get("hbm_bytes_per_algo_byte"):
+            traffic = round(rec["hbm_bytes_per_algo_byte"] * algo_bytes, -3)
     line = {
         "metric": "GP posteriors/sec (batched mixed-precision serving, p99 latency reported)",
         "value": round(value, 2),
