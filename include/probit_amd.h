@@ -274,7 +274,8 @@ int pgp_laplace_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik,
 
 /* VBGP.approximate_posterior (probit/approximators.py:332-339): plain
  * fixed-point iteration of f_VB (probit/implicit/VB.py:4-16) from zero and
- * precision = 1/sigma^2; leaves sigma^2 I + K factored in the workspace.
+ * precision = 1/sigma^2; leaves the factor of I + K / sigma^2 (what predict
+ * uses with precision 1/sigma^2) in the workspace.
  * result->objective is objective_VB (probit/implicit/VB.py:19-40), which
  * needs grad_workspace (pgp_grad_workspace_size) for the explicit inverse. */
 int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
@@ -282,6 +283,19 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
                pgp_fit_result_t* result, void* workspace,
                int64_t workspace_bytes, void* grad_workspace,
                int64_t grad_workspace_bytes, void* stream);
+
+/* d objective_VB / d(variance, lengthscale[0..D), period[0..D), noise_std,
+ * cutpoints[0..J]) at the resident VB fixed point (after pgp_vb_fit, with
+ * its workspace and gradient workspace), laid out as pgp_laplace_grad's:
+ * the value jax.value_and_grad yields for VBGP through fixed_point_layer
+ * (probit/approximators.py:132-134,316-330, probit/implicit/solvers.py:52-64,
+ * probit/implicit/VB.py:4-40).  A likelihood curvature below zero somewhere
+ * (the probit floor) gives NaN.  The resident state (the factor predict
+ * uses) is the same afterwards. */
+int pgp_vb_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik,
+                double* grad_out, void* workspace, int64_t workspace_bytes,
+                void* grad_workspace, int64_t grad_workspace_bytes,
+                void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched serving entry points                                         */

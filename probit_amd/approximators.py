@@ -312,7 +312,8 @@ class Approximator(ABC):
 
 
 class _LaplaceEvidence(torch.autograd.Function):
-    """objective_LA with its hand-written adjoint (pgp_laplace_grad)."""
+    """The objective at the fixed point (objective_LA or objective_VB) with
+    its hand-written adjoint (gp._grad: pgp_laplace_grad / pgp_vb_grad)."""
 
     @staticmethod
     def forward(ctx, gp, parameters, var, ls, per, noise, cut):
@@ -551,13 +552,36 @@ class VBGP(Approximator):
     def construct(self):
         raise NotImplementedError("VBGP.construct (f_VB as a standalone map) is not provided")
 
+    def _grad(self, ks, ls):
+        lib = _lib.lib()
+        ws = self._workspace()
+        gws = self._grad_workspace()
+        out = (np.zeros(2 + 2 * self.D + _lib.MAX_CUT)).astype(np.float64)
+        rc = lib.pgp_vb_grad(ks, ls, out.ctypes.data, ws.data_ptr(), ws.numel(),
+                             gws.data_ptr(), gws.numel(), self._stream())
+        _lib.check(rc, "pgp_vb_grad")
+        return out
+
     def objective(self):
-        """objective_VB (probit/implicit/VB.py:19-40) at the VB fixed point (value only)."""
+        """objective_VB (probit/implicit/VB.py:19-40) at the VB fixed point,
+        differentiable through torch autograd via the hand-written adjoint
+        (pgp_vb_grad: the implicit-function gradient fixed_point_layer gives,
+        probit/implicit/solvers.py:52-64)."""
 
         def obj(parameters):
-            ks, ls = self._structs(parameters)
-            res = self._run_fit(ks, ls)[2]
-            return torch.tensor(res.objective, dtype=torch.float64)
+            kern = self._kernel(parameters[0])
+            var, ls, per = kern.spec_tensors(self.D)
+            lp = parameters[1]
+            lp = lp if isinstance(lp, (tuple, list)) else (lp,)
+            noise = torch.as_tensor(lp[0], dtype=torch.float64) if not isinstance(
+                lp[0], torch.Tensor) else lp[0].to(torch.float64)
+            cut = None
+            if self.lik_kind != _lib.LIK_GAUSSIAN:
+                c = lp[1]
+                cut = c.to(torch.float64) if isinstance(c, torch.Tensor) else torch.as_tensor(
+                    np.asarray(c, dtype=np.float64))
+            return _LaplaceEvidence.apply(self, parameters, var, ls, per, noise.reshape(()),
+                                          cut)
 
         return obj
 

@@ -298,4 +298,37 @@ void launch_vb_trace(const Batch& b, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_vb_trace, dim3(b.count), dim3(NTHREADS), 0, s, b.dprobs, out);
 }
 
+// The VB adjoint's noise-std terms besides the likelihood partials:
+// out[0] = tr(C), out[1] = v . (d1 - 2 sigma a) with v in X4.
+__global__ __launch_bounds__(NTHREADS) void k_vb_sigma_terms(Prob* const* probs, double* out) {
+  const Prob& P = *probs[blockIdx.x];
+  const double* v = vslot(P, V_X4);
+  const double* a = vslot(P, V_Z);
+  const double* g = vslot(P, V_G);
+  double tr = 0.0, dv = 0.0;
+  for (int i = threadIdx.x; i < P.N; i += NTHREADS) {
+    tr += P.C[(size_t)i + (size_t)i * P.Np];
+    dv += v[i] * (g[i] - 2.0 * P.noise_std * a[i]);
+  }
+  __shared__ double r1[NTHREADS], r2[NTHREADS];
+  r1[threadIdx.x] = tr;
+  r2[threadIdx.x] = dv;
+  __syncthreads();
+  for (int st = NTHREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      r1[threadIdx.x] += r1[threadIdx.x + st];
+      r2[threadIdx.x] += r2[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = r1[0];
+    out[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
+void launch_vb_sigma_terms(const Batch& b, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_vb_sigma_terms, dim3(b.count), dim3(NTHREADS), 0, s, b.dprobs, out);
+}
+
 }  // namespace pgp

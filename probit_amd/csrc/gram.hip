@@ -183,10 +183,10 @@ __global__ __launch_bounds__(NTHREADS) void k_form_B(Prob* const* probs, Sched s
     const double kij = P.K[(size_t)i + (size_t)j * ld];
     k[c] = kij;
     double v;
-    if (vb) {
+    if (vb == 1) {  // sigma^2 I + K
       v = kij + ((i == j) ? ((i < P.N) ? s2 : 1.0) : 0.0);
-    } else {
-      v = si * kij * S[j] + ((i == j) ? 1.0 : 0.0);
+    } else {        // s I + S K S: s = 1 (B), or sigma (vb == 2, the VB adjoint's B_sigma)
+      v = si * kij * S[j] + ((i == j) ? ((vb == 2 && i < P.N) ? P.noise_std : 1.0) : 0.0);
     }
     L[(size_t)i + (size_t)j * ld] = (T)v;
   }

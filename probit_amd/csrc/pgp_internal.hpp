@@ -287,6 +287,12 @@ enum VecOp {
   OP_VB_PREC = 8,
   OP_GRAD_S2 = 9,    // X2 = -0.5 X1 .* D3
   OP_GRAD_SHAT = 10, // X3 = X2 - T
+  // VB adjoint (pgp_vb_grad)
+  OP_VBG_R = 11,     // X1 = Z - G
+  OP_VBG_T = 12,     // T = S .* X2
+  OP_VBG_Z = 13,     // X4 = S .* U
+  OP_VBG_V = 14,     // X4 = (X2 - X3) / sigma^2
+  OP_VBG_SHAT = 15,  // X3 = Z - G - sigma W .* X4, X1 = 0, X2 = sigma X4
 };
 enum RedWhat { RED_STEP = 0, RED_FINAL = 1 };
 void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s);

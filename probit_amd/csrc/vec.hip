@@ -234,6 +234,26 @@ __global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, Sched sc, 
     case OP_GRAD_SHAT:
       vslot(P, V_X3)[i] = vslot(P, V_X2)[i] - vslot(P, V_T)[i];
       break;
+    case OP_VBG_R:
+      vslot(P, V_X1)[i] = vslot(P, V_Z)[i] - vslot(P, V_G)[i];
+      break;
+    case OP_VBG_T:
+      vslot(P, V_T)[i] = vslot(P, V_S)[i] * vslot(P, V_X2)[i];
+      break;
+    case OP_VBG_Z:
+      vslot(P, V_X4)[i] = vslot(P, V_S)[i] * vslot(P, V_U)[i];
+      break;
+    case OP_VBG_V:
+      vslot(P, V_X4)[i] =
+          (vslot(P, V_X2)[i] - vslot(P, V_X3)[i]) / (P.noise_std * P.noise_std);
+      break;
+    case OP_VBG_SHAT: {
+      const double v = vslot(P, V_X4)[i];
+      vslot(P, V_X3)[i] = vslot(P, V_Z)[i] - vslot(P, V_G)[i] - P.noise_std * vslot(P, V_W)[i] * v;
+      vslot(P, V_X1)[i] = 0.0;
+      vslot(P, V_X2)[i] = P.noise_std * v;
+      break;
+    }
     case OP_VB_PREC: {
       const double w = (i < P.N) ? 1.0 / (P.noise_std * P.noise_std) : 0.0;
       vslot(P, V_W)[i] = w;
