@@ -292,6 +292,15 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
  * probit/implicit/VB.py:4-40).  A likelihood curvature below zero somewhere
  * (the probit floor) gives NaN.  The resident state (the factor predict
  * uses) is the same afterwards. */
+/* The VB fixed-point map f_VB(w) = (sigma^2 I + K)^-1 (K w + sigma
+ * d log p(y | K w) / df) into problem->weight (probit/implicit/VB.py:4-16,
+ * VBGP.construct, probit/approximators.py:306-314).  Leaves sigma^2 I + K
+ * factored in the workspace (a later predict re-forms its own factor). */
+int pgp_vb_map(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+               const pgp_likelihood_t* lik, int32_t precision,
+               const double* weight, void* workspace, int64_t workspace_bytes,
+               void* stream);
+
 int pgp_vb_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik,
                 double* grad_out, void* workspace, int64_t workspace_bytes,
                 void* grad_workspace, int64_t grad_workspace_bytes,

@@ -150,6 +150,8 @@ _SIGNATURES = {
                                          _I64, _P]),
     "pgp_laplace_grad": (C.c_int, [C.POINTER(KernelT), C.POINTER(LikelihoodT), _P, _P, _I64, _P,
                                    _I64, _P]),
+    "pgp_vb_map": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT), C.POINTER(LikelihoodT),
+                             C.c_int32, _P, _P, _I64, _P]),
     "pgp_vb_grad": (C.c_int, [C.POINTER(KernelT), C.POINTER(LikelihoodT), _P, _P, _I64, _P,
                                    _I64, _P]),
     "pgp_vb_fit": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT), C.POINTER(LikelihoodT),

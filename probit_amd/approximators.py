@@ -550,7 +550,23 @@ class VBGP(Approximator):
         return w, p
 
     def construct(self):
-        raise NotImplementedError("VBGP.construct (f_VB as a standalone map) is not provided")
+        """f_VB: (parameters, weight) -> (sigma^2 I + K)^-1 (K weight + sigma
+        d log p(y | K weight)) (probit/approximators.py:306-314,
+        probit/implicit/VB.py:4-16)."""
+
+        def f(parameters, weight):
+            ks, ls = self._structs(parameters)
+            lib = _lib.lib()
+            ws = self._workspace()
+            w = self._as_device_vec(weight)
+            out = self._vec()
+            rc = lib.pgp_vb_map(self._problem(out), ks, ls, self._prec, w.data_ptr(),
+                                ws.data_ptr(), ws.numel(), self._stream())
+            _lib.check(rc, "pgp_vb_map")
+            self._state = None  # the workspace now holds sigma^2 I + K
+            return out
+
+        return f
 
     def _grad(self, ks, ls):
         lib = _lib.lib()

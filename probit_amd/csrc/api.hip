@@ -664,6 +664,38 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   return 0;
 }
 
+int pgp_vb_map(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+               const pgp_likelihood_t* lik, int32_t precision, const double* weight,
+               void* workspace, int64_t workspace_bytes, void* stream) {
+  int rc = validate(problem, kernel, lik);
+  if (rc) return rc;
+  if (!lik) return fail(-3, "likelihood is NULL");
+  if (precision < 0 || precision > 2) return fail(-4, "unknown precision %d", precision);
+  if (!weight) return fail(-5, "weight is NULL");
+  if (!problem->weight) return fail(-1, "problem->weight (the output) is NULL");
+  const WsLayout L = ws_layout(problem->N, precision);
+  if (!workspace || workspace_bytes < L.total) return fail(-6, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  Prob p = make_prob(problem, kernel, lik, precision, ws);
+  p.out_prec = nullptr;
+  p.out_latent = nullptr;
+  write_prob(ws, p, 1, s);
+  set_self(ws, s);
+  const Batch b = single_batch(ws, p.Np);
+  launch_gram_self(b, s);
+  launch_load_state(b, weight, nullptr, s);
+  launch_gemv_sym(b, V_Z, V_F, 0, s);  // f = K w
+  launch_lik_final(b, 0, s);           // d1 at f
+  launch_vec_op(b, OP_VB_RHS, s);      // X1 = f + sigma d1
+  launch_copy_slot(b, V_X1, V_T, s);
+  factor(b, precision, 1, s);          // sigma^2 I + K
+  solve_B(b, precision, s);            // U = (sigma^2 I + K)^-1 X1
+  launch_copy_slot(b, V_U, V_Z, s);
+  launch_vec_op(b, OP_OUT, s);         // problem->weight <- U
+  return check_launch();
+}
+
 int pgp_laplace_set_state(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
                           const pgp_likelihood_t* lik, int32_t precision, const double* weight,
                           const double* W, pgp_fit_result_t* result, void* workspace,

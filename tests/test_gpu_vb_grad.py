@@ -74,3 +74,30 @@ def test_vb_gradient_keeps_the_predict_state():
     gp._grad(*gp._structs((theta, lp)))
     m1, v1 = gp.predict(Xt, (theta, lp), w, p)
     assert np.array_equal(m1.cpu().numpy(), m0) and np.array_equal(v1.cpu().numpy(), v0)
+
+
+@pytest.mark.parametrize("lik", ["gaussian", "probit"])
+def test_vb_construct_is_the_oracle_map(lik):
+    """VBGP.construct() is f_VB (probit/implicit/VB.py:4-16) at any weight, and
+    the converged weight is its fixed point; predict afterwards is unchanged."""
+    import probit_amd as P
+    # sigma >= 0.5: the VB fixed-point iteration contracts (its iteration
+    # matrix has eigenvalues up to |1 - 1/sigma| for a Gaussian likelihood)
+    X, y, theta, lp, spec = small_problem(43, 130, 2, lik, "Matern12", noise=0.6)
+    ll = P.log_gaussian_likelihood if lik == "gaussian" else P.log_probit_likelihood
+    gp = P.VBGP(data=(X, y), prior=make_prior("Matern12"), log_likelihood=ll, tolerance=1e-11,
+                maxiter=100000)
+    f = gp.construct()
+    w = np.random.default_rng(44).normal(size=130)
+    got = f((theta, lp), w).cpu().numpy()
+    ref = O.f_VB(O.gram(spec, X), y, lik_kind(lik), lp, w)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+    ws, ps = gp.approximate_posterior((theta, lp))
+    Xt = np.random.default_rng(45).normal(size=(20, 2))
+    m0, v0 = gp.predict(Xt, (theta, lp), ws, ps)
+    m0, v0 = m0.cpu().numpy().copy(), v0.cpu().numpy().copy()
+    fw = f((theta, lp), ws).cpu().numpy()
+    assert np.max(np.abs(fw - ws.cpu().numpy())) <= 1e-9 * np.max(np.abs(fw))
+    m1, v1 = gp.predict(Xt, (theta, lp), ws, ps)
+    assert np.allclose(m1.cpu().numpy(), m0, rtol=1e-12, atol=0)
+    assert np.allclose(v1.cpu().numpy(), v0, rtol=1e-12, atol=0)
