@@ -12,13 +12,15 @@ from .kernels import EQ, Kernel, Matern12
 from .optimize import minimise_l_bfgs_b
 from .utilities import (BOUNDS, CutpointValueError, InvalidKernel, check_cutpoints, check_data,
                         grad_log_probit_likelihood, h, hessian_log_probit_likelihood,
-                        log_gaussian_likelihood, log_probit_likelihood,
-                        probit_predictive_distributions)
+                        log_gaussian_likelihood, log_probit_likelihood, ndtr, norm_cdf,
+                        norm_logpdf, norm_pdf, norm_z_logpdf, norm_z_pdf, probit,
+                        probit_likelihood, probit_predictive_distributions)
 
 __all__ = [
     "Approximator", "LaplaceGP", "VBGP", "EQ", "Matern12", "Kernel", "BOUNDS",
     "CutpointValueError", "InvalidKernel", "check_cutpoints", "check_data",
     "grad_log_probit_likelihood", "hessian_log_probit_likelihood", "h",
     "log_gaussian_likelihood", "log_probit_likelihood", "probit_predictive_distributions",
-    "minimise_l_bfgs_b",
+    "ndtr", "norm_cdf", "norm_pdf", "norm_z_pdf", "norm_logpdf", "norm_z_logpdf", "probit",
+    "probit_likelihood", "minimise_l_bfgs_b",
 ]

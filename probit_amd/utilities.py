@@ -30,6 +30,14 @@ __all__ = [
     "probit_predictive_distributions",
     "check_data",
     "check_cutpoints",
+    "ndtr",
+    "norm_cdf",
+    "norm_pdf",
+    "norm_z_pdf",
+    "norm_logpdf",
+    "norm_z_logpdf",
+    "probit",
+    "probit_likelihood",
     "CutpointValueError",
     "InvalidKernel",
 ]
@@ -44,6 +52,77 @@ def h(x):
     x = np.asarray(x, dtype=np.float64)
     with np.errstate(divide="ignore", invalid="ignore"):
         return -1.0 * x**-2 + 5.0 / 2.0 * x**-4 - 37.0 / 3.0 * x**-6
+
+
+# --------------------------------------------------------------------------
+# elementwise normal / probit helpers (probit/utilities.py:18-34,47-53,64-70,
+# 195-229): the reference's formulas on float64 tensors, on the device of
+# their input.  The Newton / predict path does not call them; it evaluates
+# the same quantities inside the fused HIP likelihood kernels.
+# --------------------------------------------------------------------------
+_OVER_SQRT_2PI = 1.0 / math.sqrt(2.0 * math.pi)
+
+
+def _t(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(torch.float64)
+    return torch.as_tensor(np.asarray(x, dtype=np.float64))
+
+
+def ndtr(z):
+    """Standard normal CDF as 0.5 (1 + erf(z / sqrt 2)) (probit/utilities.py:18-19)."""
+    return 0.5 * (1.0 + torch.erf(_t(z) / math.sqrt(2.0)))
+
+
+def norm_z_pdf(z):
+    """probit/utilities.py:22-23."""
+    z = _t(z)
+    return _OVER_SQRT_2PI * torch.exp(-0.5 * z * z)
+
+
+def norm_pdf(x, loc=0.0, scale=1.0):
+    """probit/utilities.py:26-28."""
+    return norm_z_pdf((_t(x) - loc) / scale) / scale
+
+
+def norm_cdf(x):
+    """ndtr with +-inf mapped to 1 / 0 (probit/utilities.py:31-34)."""
+    x = _t(x)
+    inf = torch.isinf(x)
+    safe = torch.where(inf, torch.ones_like(x), x)
+    return torch.where(inf, (x == math.inf).to(x.dtype), ndtr(safe))
+
+
+def norm_z_logpdf(x):
+    """probit/utilities.py:64-65."""
+    x = _t(x)
+    return math.log(_OVER_SQRT_2PI) - x * x / 2.0
+
+
+def norm_logpdf(x, loc=0.0, scale=1.0):
+    """probit/utilities.py:68-70."""
+    return norm_z_logpdf((_t(x) - loc) / scale) - math.log(scale)
+
+
+def probit(noise_std, cutpoints_y, cutpoints_yplus1, f):
+    """Z = Phi((b_{y+1} - f) / s) - Phi((b_y - f) / s) with the infinite
+    cutpoints guarded (probit/utilities.py:195-229)."""
+    f = _t(f)
+    b1 = _t(cutpoints_y).to(f.device)
+    b2 = _t(cutpoints_yplus1).to(f.device)
+    z1 = torch.where(b1 == -math.inf, torch.zeros_like(f), b1 - f)
+    z2 = torch.where(b2 == math.inf, torch.zeros_like(f), b2 - f)
+    c1 = torch.where(b1 == -math.inf, torch.zeros_like(f), norm_cdf(z1 / noise_std))
+    c2 = torch.where(b2 == math.inf, torch.ones_like(f), norm_cdf(z2 / noise_std))
+    return c2 - c1
+
+
+def probit_likelihood(f, y, likelihood_parameters):
+    """probit(noise_std, cutpoints[y], cutpoints[y + 1], f) (probit/utilities.py:47-53)."""
+    f = _t(f)
+    cut = _t(likelihood_parameters[1]).to(f.device)
+    yi = _t(y).to(f.device).long()
+    return probit(_scalar(likelihood_parameters[0]), cut[yi], cut[yi + 1], f)
 
 
 # --------------------------------------------------------------------------
