@@ -64,6 +64,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--seed", type=int, default=4)
     p.add_argument("--stream", type=int, default=4096)
+    p.add_argument("--min-n", type=int, default=0,
+                   help="serve only the stream's requests with at least this many points "
+                        "(0: the whole stream)")
     return p.parse_args()
 
 
@@ -215,6 +218,11 @@ def main():
     R, K, W = args.requests_per_step, args.steps, args.warmup
     total = (K + W) * R
     stream = data.serving_stream(seed=args.seed, n=min(args.stream, total))
+    if args.min_n > 0:
+        stream = [r for r in data.serving_stream(seed=args.seed, n=args.stream)
+                  if r.problem.N >= args.min_n]
+        if not stream:
+            sys.exit(f"bench.py: no request of the stream has N >= {args.min_n}")
     order = [stream[i % len(stream)] for i in range(total)]
     dev_reqs = upload(order, torch)
     # distinct request ids even when the stream is cycled
@@ -306,6 +314,7 @@ def main():
             "requests_per_step": R,
             "capacity": args.capacity,
             "lane_np": os.environ.get("PGP_LANE_NP", "1024"),
+            "min_n": args.min_n,
             "precision": args.precision,
             "tolerance": 1e-5,
         },
