@@ -336,6 +336,35 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, 
   }
 }
 
+// The trailing updates of FP32 factors (outer update, column update, column
+// kernel) stage their operands m-major for mma64_f32_mmajor; FP64 factors
+// stay k-major and BF16 operands are m-major bf16.  All three kernels use
+// the same form, so a tile's update is bitwise the same in each.
+template <typename T, int BF16>
+struct MmajorF32 {
+  static constexpr bool value = std::is_same<T, float>::value && !BF16;
+};
+
+// k index of the q-th of the 16 operand values a thread (row m, group kb)
+// fetches per 64-column chunk: 16 consecutive k for the m-major FP32 form,
+// every fourth k otherwise.  Either way one fetch instruction reads one
+// column's 64 consecutive rows.
+template <bool MM>
+__device__ __forceinline__ int operand_k(int kb, int q) {
+  return MM ? 16 * kb + q : kb + 4 * q;
+}
+
+// d[m * pitch + 16 kb + q] = v[q], q = 0..15, as four 16-byte stores
+// (pitch a multiple of 4).
+template <typename T>
+__device__ __forceinline__ void stash_f32_mmajor(T* d, int pitch, const T (&v)[16], int m, int kb) {
+  static_assert(std::is_same<T, float>::value, "m-major FP32 operands only");
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    *(floatx4*)&d[m * pitch + 16 * kb + 4 * j] =
+        floatx4{v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]};
+}
+
 // Workgroups of one launch are dealt to the 8 XCDs round-robin (workgroup g
 // runs on XCD g % 8).  Renumber them so that each XCD gets one contiguous
 // run of tiles: neighbouring tiles share panel rows, which then stay in that
@@ -352,11 +381,13 @@ __device__ __forceinline__ int xcd_contiguous(int g, int total) {
 // and written once however long the K range.
 // FUSE: the workgroup given factor = true (the tile that becomes the next
 // diagonal block) keeps its updated tile in LDS and factors it at once.
+// Requires kc1 > kc0 (both callers update by at least one block column).
 template <typename T, int BF16, int FUSE>
 __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj,
                                 bool factor, const FactorOut& o) {
   auto A = gbl(A_);
-  constexpr int PITCH = BF16 ? 72 : KP<T>::value;
+  constexpr bool MM = MmajorF32<T, BF16>::value;
+  constexpr int PITCH = BF16 ? 72 : MM ? 68 : KP<T>::value;
   typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
   constexpr int OPER = 2 * TILE * PITCH * (int)sizeof(LT);
   constexpr int BLK = FUSE ? 2 * TILE * (TILE + 1) * 8 : 0;
@@ -372,13 +403,50 @@ __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int 
     const int64_t c0 = (int64_t)c * TILE;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int k = kb + 4 * q;
+      const int k = operand_k<MM>(kb, q);
       ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
       rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
     }
   };
   fetch(kc0);
-  // the C tile is loaded up front, so its latency overlaps the product
+  auto stash = [&](LT* da, LT* db) {
+    if constexpr (MM) {
+      stash_f32_mmajor(da, PITCH, ra, m, kb);
+      stash_f32_mmajor(db, PITCH, rb, m, kb);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = kb + 4 * q;
+        if constexpr (BF16) {
+          da[m * PITCH + k] = f2bf((float)ra[q]);
+          db[m * PITCH + k] = f2bf((float)rb[q]);
+        } else {
+          da[k * PITCH + m] = ra[q];
+          db[k * PITCH + m] = rb[q];
+        }
+      }
+    }
+  };
+  auto mma = [&](const LT* da, const LT* db) {
+    if constexpr (BF16)
+      mma64_bf16(da, PITCH, db, PITCH, acc);
+    else if constexpr (MM)
+      mma64_f32_mmajor(da, PITCH, db, PITCH, acc);
+    else
+      mma64_kmajor<T>(da, PITCH, db, PITCH, acc);
+  };
+  for (int c = kc0; c < kc1 - 1; ++c) {
+    if (c > kc0) __syncthreads();  // the previous chunk's products are done with LDS
+    stash(sa, sb);
+    __syncthreads();
+    fetch(c + 1);
+    mma(sa, sb);
+  }
+  if (kc1 - 1 > kc0) __syncthreads();
+  stash(sa, sb);
+  __syncthreads();
+  // the C tile is loaded during the last chunk's product (the fetch
+  // registers are free then), so its latency overlaps the product
   T cpre[2][2][4];
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
@@ -387,32 +455,7 @@ __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int 
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         cpre[bm][bn][r] = A[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(rj + acc_col(bn)) * ld];
-  auto stash = [&](LT* da, LT* db) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = kb + 4 * q;
-      if constexpr (BF16) {
-        da[m * PITCH + k] = f2bf((float)ra[q]);
-        db[m * PITCH + k] = f2bf((float)rb[q]);
-      } else {
-        da[k * PITCH + m] = ra[q];
-        db[k * PITCH + m] = rb[q];
-      }
-    }
-  };
-  auto mma = [&](const LT* da, const LT* db) {
-    if constexpr (BF16)
-      mma64_bf16(da, PITCH, db, PITCH, acc);
-    else
-      mma64_kmajor<T>(da, PITCH, db, PITCH, acc);
-  };
-  for (int c = kc0; c < kc1; ++c) {
-    if (c > kc0) __syncthreads();  // the previous chunk's products are done with LDS
-    stash(sa, sb);
-    __syncthreads();
-    if (c + 1 < kc1) fetch(c + 1);
-    mma(sa, sb);
-  }
+  mma(sa, sb);
   if constexpr (!FUSE) factor = false;
   double (*blk)[TILE + 1] = (double (*)[TILE + 1])arena;
   if (factor) __syncthreads();  // operands are dead; the arena becomes the block
@@ -514,7 +557,8 @@ template <typename T, int BF16, typename OT>
 __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int kc0, int kc1, int64_t ri,
                                    int64_t rj, char* arena, OT* out, int opitch, bool kmajor) {
   auto A = gbl(A_);
-  constexpr int PITCH = BF16 ? 72 : KP<T>::value;
+  constexpr bool MM = MmajorF32<T, BF16>::value;
+  constexpr int PITCH = BF16 ? 72 : MM ? 68 : KP<T>::value;
   typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
   typedef typename Acc<T>::type acc_t;
   LT* sa = (LT*)arena;
@@ -528,7 +572,7 @@ __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int 
       const int64_t c0 = (int64_t)c * TILE;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int k = kb + 4 * q;
+        const int k = operand_k<MM>(kb, q);
         ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
         rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
       }
@@ -536,21 +580,28 @@ __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int 
     fetch(kc0);
     for (int c = kc0; c < kc1; ++c) {
       if (c > kc0) __syncthreads();
+      if constexpr (MM) {
+        stash_f32_mmajor(sa, PITCH, ra, m, kb);
+        stash_f32_mmajor(sb, PITCH, rb, m, kb);
+      } else {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int k = kb + 4 * q;
-        if constexpr (BF16) {
-          sa[m * PITCH + k] = f2bf((float)ra[q]);
-          sb[m * PITCH + k] = f2bf((float)rb[q]);
-        } else {
-          sa[k * PITCH + m] = ra[q];
-          sb[k * PITCH + m] = rb[q];
+        for (int q = 0; q < 16; ++q) {
+          const int k = kb + 4 * q;
+          if constexpr (BF16) {
+            sa[m * PITCH + k] = f2bf((float)ra[q]);
+            sb[m * PITCH + k] = f2bf((float)rb[q]);
+          } else {
+            sa[k * PITCH + m] = ra[q];
+            sb[k * PITCH + m] = rb[q];
+          }
         }
       }
       __syncthreads();
       if (c + 1 < kc1) fetch(c + 1);
       if constexpr (BF16)
         mma64_bf16(sa, PITCH, sb, PITCH, acc);
+      else if constexpr (MM)
+        mma64_f32_mmajor(sa, PITCH, sb, PITCH, acc);
       else
         mma64_kmajor<T>(sa, PITCH, sb, PITCH, acc);
     }

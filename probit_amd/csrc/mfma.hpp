@@ -88,6 +88,41 @@ __device__ __forceinline__ void mma64_kmajor(const T* As, int pa, const T* Bs, i
   }
 }
 
+// acc += A B^T over K = 64 for float, m-major LDS operands As[m * pa + k],
+// Bs[n * pb + k] (pa, pb multiples of 4; pa = pb = 4 mod 32 makes every
+// 16-byte read conflict-free).  MFMA step s gives lane group q = l >> 4 the
+// k index 16 q + s, so a lane's operands for a half of the chunk are 8
+// consecutive floats: two 16-byte LDS reads per operand and half, against
+// one 4-byte read per operand and step in the k-major form.  The k order
+// inside each MFMA differs from mma64_kmajor's, so every kernel whose
+// results must agree bit for bit uses the same one of the two.
+__device__ __forceinline__ void mma64_f32_mmajor(const float* As, int pa, const float* Bs, int pb,
+                                                 floatx4 (&acc)[2][2]) {
+  const int l = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int wr = 32 * (w >> 1), wc = 32 * (w & 1);
+  const int lr = l & 15, k0 = 16 * (l >> 4);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    floatx4 a0[2], a1[2], b0[2], b1[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int k = k0 + 8 * h + 4 * j;
+      a0[j] = *(const floatx4*)&As[(wr + lr) * pa + k];
+      a1[j] = *(const floatx4*)&As[(wr + 16 + lr) * pa + k];
+      b0[j] = *(const floatx4*)&Bs[(wc + lr) * pb + k];
+      b1[j] = *(const floatx4*)&Bs[(wc + 16 + lr) * pb + k];
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      acc[0][0] = mfma16(a0[s >> 2][s & 3], b0[s >> 2][s & 3], acc[0][0]);
+      acc[0][1] = mfma16(a0[s >> 2][s & 3], b1[s >> 2][s & 3], acc[0][1]);
+      acc[1][0] = mfma16(a1[s >> 2][s & 3], b0[s >> 2][s & 3], acc[1][0]);
+      acc[1][1] = mfma16(a1[s >> 2][s & 3], b1[s >> 2][s & 3], acc[1][1]);
+    }
+  }
+}
+
 // acc += A B^T-style product for bf16: As[m * pa + k], Bs[n * pb + k]
 // (m-major / n-major bf16 bits), K = 64 as two x32 steps.
 __device__ __forceinline__ void mma64_bf16(const unsigned short* As, int pa,
