@@ -89,8 +89,9 @@ __device__ __forceinline__ void mma64_kmajor(const T* As, int pa, const T* Bs, i
 }
 
 // acc += A B^T over K = 64 for float, m-major LDS operands As[m * pa + k],
-// Bs[n * pb + k] (pa, pb multiples of 4; pa = pb = 4 mod 32 makes every
-// 16-byte read conflict-free).  MFMA step s gives lane group q = l >> 4 the
+// Bs[n * pb + k] (pa, pb multiples of 4; with pa = pb = 4 mod 32 the 16-byte
+// reads of 8 consecutive lanes cover the 32 banks once; the counters still
+// show a few conflict cycles).  MFMA step s gives lane group q = l >> 4 the
 // k index 16 q + s, so a lane's operands for a half of the chunk are 8
 // consecutive floats: two 16-byte LDS reads per operand and half, against
 // one 4-byte read per operand and step in the k-major form.  The k order
