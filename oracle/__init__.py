@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY. Only ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
-checker (or as the timed CPU baseline), never as the thing measured or shipped.
+reference that results are compared against (or as the timed CPU baseline),
+never as the thing measured or shipped.
 The product package ``probit_amd`` never imports it.
 
 ``gp_oracle`` restates, in NumPy/SciPy float64, the reference's
