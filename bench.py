@@ -194,7 +194,7 @@ def cpu_baseline(reqs, seconds):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
+        print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); the "
               "serving path shards by routing requests to independent per-GPU engines, "
               "which is not run here", file=sys.stderr)
         sys.exit(2)
