@@ -12,7 +12,8 @@ Extra keyword-only arguments choose the factor precision ("fp64", "fp32",
 "bf16"), the Newton iteration cap and the number of FP64 refinement steps.
 All compute runs in libprobit_amd (HIP, gfx950); torch supplies device
 memory, streams and the autograd graph around the evidence.  Numerical
-failures surface as NaN (as JAX would produce) plus ``last_result.status``.
+failures surface as NaN (as JAX would produce) plus ``last_result.status``;
+with ``strict=True`` they raise ``FloatingPointError`` instead.
 """
 
 from __future__ import annotations
@@ -74,6 +75,23 @@ def _unflatten(spec, leaves):
     return rec(spec)
 
 
+# status bits whose results are not valid numbers (INFO_ESCALATED only
+# reports that BF16 updates were replaced by FP32 ones)
+_FAILURES = (("non-positive pivot", _lib.INFO_NOT_PD),
+             ("negative precision W", _lib.INFO_NEGATIVE_W),
+             ("fixed point not reached within maxiter", _lib.INFO_MAXITER),
+             ("a solve's bounded wait ran out", _lib.INFO_STALLED))
+
+
+def numerical_failure(status: int, objective: float = 0.0) -> str | None:
+    """What makes a fit result invalid, or None.  The reference returns NaN
+    (JAX) and raises nothing; ``strict=True`` raises with this text."""
+    names = [name for name, bit in _FAILURES if status & bit]
+    if not np.isfinite(objective):
+        names.append("non-finite objective")
+    return ", ".join(names) or None
+
+
 class Approximator(ABC):
     """Base class: data upload, parameter packing, workspaces, predict."""
 
@@ -83,8 +101,9 @@ class Approximator(ABC):
 
     def __init__(self, data, prior, log_likelihood, grad_log_likelihood=None,
                  hessian_log_likelihood=None, tolerance=1e-5, *, precision_mode="fp64",
-                 maxiter=100, refine=None, device=None):
+                 maxiter=100, refine=None, device=None, strict=False):
         self.tolerance = float(tolerance)
+        self.strict = bool(strict)
         self.prior = prior
         self.lik_kind = _resolve_likelihood(log_likelihood, grad_log_likelihood,
                                             hessian_log_likelihood)
@@ -174,6 +193,14 @@ class Approximator(ABC):
     def _vec(self):
         return torch.empty(self.N, dtype=torch.float64, device=self.device)
 
+    def _check_status(self, res, what):
+        """strict=True: raise on a fit or state whose numbers are not valid."""
+        if not self.strict:
+            return
+        msg = numerical_failure(res.status, res.objective)
+        if msg:
+            raise FloatingPointError(f"{what}: {msg} (status {res.status})")
+
     # ---- resident-state bookkeeping -----------------------------------------
     def _remember(self, key, weight, precision, result):
         self._state = dict(key=key, weight=weight, precision=precision,
@@ -209,6 +236,7 @@ class Approximator(ABC):
         _lib.check(rc, "pgp_laplace_set_state")
         Wt = W if W is not None else Wout
         self._remember(self._key(ks, ls), w, Wt, res)
+        self._check_status(res, "pgp_laplace_set_state")
         return w, Wt, fout, res
 
     # ---- public API ---------------------------------------------------------
@@ -436,6 +464,7 @@ class LaplaceGP(Approximator):
                     _lib.check(lib.pgp_batcher_step(h, done, n, C.byref(nd)), "pgp_batcher_step")
                     finished += nd.value
                 for it in items:
+                    self._check_status(it.result, "batched pgp_laplace_fit")
                     if it.result.status & _lib.INFO_MAXITER:
                         warnings.warn(f"Newton iteration stopped at maxiter={self.maxiter}")
                 out.extend(k[1] for k in keep)
@@ -452,6 +481,7 @@ class LaplaceGP(Approximator):
                                  ws.data_ptr(), ws.numel(), self._stream())
         _lib.check(rc, "pgp_laplace_fit")
         self._remember(self._key(ks, ls), weight, precision, res)
+        self._check_status(res, "pgp_laplace_fit")
         if res.status & _lib.INFO_MAXITER:
             warnings.warn(f"Newton iteration stopped at maxiter={self.maxiter} "
                           f"(step {res.step_norm:.3e} > tol {self.tolerance:.1e})")
@@ -540,6 +570,7 @@ class VBGP(Approximator):
                             self._stream())
         _lib.check(rc, "pgp_vb_fit")
         self._remember(self._key(ks, ls), weight, precision, res)
+        self._check_status(res, "pgp_vb_fit")
         if res.status & _lib.INFO_MAXITER:
             warnings.warn(f"fixed-point iteration stopped at maxiter={self.maxiter}")
         return weight, precision, res

@@ -126,3 +126,27 @@ def test_edge_requests_in_one_serving_batch(mode):
         assert np.array_equal(c.weight.cpu().numpy(), w)
         assert np.array_equal(c.mean.cpu().numpy(), m)
         assert np.array_equal(c.var.cpu().numpy(), v)
+
+
+def test_strict_mode_raises_where_the_default_warns():
+    """maxiter = 1 on a probit problem: the default keeps the reference's
+    behaviour (a result, MAXITER in last_result.status, a warning); strict=True
+    raises FloatingPointError; a converged fit is the same either way."""
+    import warnings
+
+    import probit_amd as P
+    X, y, theta, lp, _ = small_problem(61, 120, 2, "probit", "EQ")
+    kw = dict(data=(X, y), prior=make_prior("EQ"), log_likelihood=P.log_probit_likelihood)
+    print("strict mode", flush=True)
+    gp = P.LaplaceGP(**kw, maxiter=1)
+    with warnings.catch_warnings(record=True) as wlist:
+        warnings.simplefilter("always")
+        w, p = gp.approximate_posterior((theta, lp))
+    assert gp.last_result.status & P._lib.INFO_MAXITER
+    assert any("maxiter" in str(x.message) for x in wlist)
+    assert np.all(np.isfinite(w.cpu().numpy()))
+    with pytest.raises(FloatingPointError, match="maxiter"):
+        P.LaplaceGP(**kw, maxiter=1, strict=True).approximate_posterior((theta, lp))
+    w0, _ = P.LaplaceGP(**kw).approximate_posterior((theta, lp))
+    w1, _ = P.LaplaceGP(**kw, strict=True).approximate_posterior((theta, lp))
+    assert np.array_equal(w0.cpu().numpy(), w1.cpu().numpy())
