@@ -16,12 +16,20 @@ latency is measured from a request's admission to its completion.  All
 request inputs are resident in HBM before timing starts.  Also measured,
 outside the timed region:
   * the plain per-request FP64 path (pgp_laplace_fit + pgp_predict, one
-    request at a time) on a sample of the same requests, and the maximum
-    relative predictive-mean error of the batched path against it;
-  * open-loop latency (Poisson arrivals) p50 / p99;
+    request at a time) on the same requests: every distinct request of the
+    timed set, its time weighted by how often the set holds it (the set
+    cycles the stream), and the maximum relative predictive-mean error of the
+    batched path against it;
+  * per-class launch times (outer update, per-column steps, TRSV, Gram,
+    TRSM) over a strided window of the timed requests served again with every
+    launch timer on;
+  * open-loop latency (the timed requests with Poisson arrivals at 0.7 x the
+    measured throughput, from arrival to completion) p50 / p99;
   * the trailing-update kernel's roofline from HIP events over the timed
-    region (pgp_timer_*), and the CPU oracle baseline on a bounded sample.
-Prints ONE JSON line.
+    region (pgp_timer_*), and the CPU oracle baseline on a stratified sample
+    of the timed requests (one per equal-count N-stratum).
+--cpu-study REPEATS times only the CPU baseline plan (both oracle variants,
+all cores and one core, medians).  Prints ONE JSON line.
 """
 
 from __future__ import annotations
@@ -57,11 +65,22 @@ def parse():
                         "reference of the HBM traffic counters)")
     p.add_argument("--refine", type=int, default=-1,
                    help="FP64 refinement steps per Newton solve (-1: library default)")
-    p.add_argument("--plain-sample", type=int, default=128)
-    p.add_argument("--open-loop", type=int, default=512, help="requests in the latency run")
+    p.add_argument("--plain-sample", type=int, default=0,
+                   help="requests of the plain FP64 leg: 0 = every distinct request of the "
+                        "timed set (weighted by how often the set holds it), else a strided "
+                        "sample of that many")
+    p.add_argument("--profile-window", type=int, default=1024,
+                   help="timed requests (strided) served again with every launch-timer class "
+                        "on, for the per-class time shares (0: skip)")
+    p.add_argument("--open-loop", type=int, default=4096,
+                   help="requests of the open-loop (Poisson arrivals) latency run")
     p.add_argument("--rate", type=float, default=0.0,
                    help="open-loop arrival rate (req/s); 0 = 0.7 x measured throughput")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--cpu-strata", type=int, default=12,
+                   help="CPU baseline: requests timed, one per equal-count N-stratum")
+    p.add_argument("--cpu-study", type=int, default=0, metavar="REPEATS",
+                   help="only time the CPU baseline plan (both oracle variants, all cores and "
+                        "one, median of REPEATS runs) and print it")
     p.add_argument("--seed", type=int, default=4)
     p.add_argument("--stream", type=int, default=4096)
     p.add_argument("--min-n", type=int, default=0,
@@ -118,15 +137,14 @@ def serve_window(engine, window, t_arrive=None, chunk=128):
 
 
 def plain_fp64(sample, torch, _lib):
-    """One request at a time through the single-problem entry points (FP64)."""
+    """One request at a time through the single-problem entry points (FP64).
+    Returns {sid: (mean, var, iterations)} and {sid: seconds}."""
     from probit_amd.kernels import EQ, Matern12
     from probit_amd.utilities import likelihood_struct
     lib = _lib.lib()
     stream = _lib.stream_handle()
-    out = {}
-    lat = []
+    out, secs = {}, {}
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     for r in sample:
         ts = time.perf_counter()
         p = r["p"]
@@ -148,47 +166,108 @@ def plain_fp64(sample, torch, _lib):
         res = _lib.FitResultT()
         _lib.check(lib.pgp_laplace_fit(prob, ks, ls, opts, res, ws.data_ptr(), ws.numel(),
                                        stream), "fit")
-        _lib.check(lib.pgp_predict(ks, r["Xt"].data_ptr(), M, mean.data_ptr(), var.data_ptr(),
-                                   ws.data_ptr(), ws.numel(), pws.data_ptr(), pws.numel(),
-                                   stream), "predict")
+        _lib.info_bits(lib.pgp_predict(ks, r["Xt"].data_ptr(), M, mean.data_ptr(),
+                                       var.data_ptr(), ws.data_ptr(), ws.numel(),
+                                       pws.data_ptr(), pws.numel(), stream), "predict")
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - ts)
-        out[r["rid"]] = (mean, var, res.iterations)
-    t1 = time.perf_counter()
-    return out, len(sample) / (t1 - t0), np.array(lat)
+        secs[r["sid"]] = time.perf_counter() - ts
+        out[r["sid"]] = (mean, var, res.iterations)
+    return out, secs
 
 
-def cpu_baseline(reqs, seconds):
-    """Reference-faithful oracle (dense Jacobian + LU Newton, dense predict
-    solve) on the host, over the stream in order until `seconds` elapse."""
+def _cpu_request_seconds(p, variant):
+    """One posterior on the host with the oracle: 'lu' = reference-faithful
+    (dense Jacobian + LU Newton per step, dense predict solve), 'bform' = the
+    Cholesky B-form Newton."""
     from oracle import gp_oracle as O
-    try:
-        from threadpoolctl import threadpool_info
-        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:  # pragma: no cover
-        cores = os.cpu_count() or 1
-    done = 0
     t0 = time.perf_counter()
-    nsum = 0
-    for r in reqs:
-        p = r.problem
-        kind = O.EQ if p.kind == "EQ" else O.MATERN12
-        spec = O.KernelSpec.make(kind, p.variance, p.lengthscale, p.D)
-        lik = O.GAUSSIAN if p.lik == "gaussian" else O.PROBIT
-        lp = p.likelihood_parameters()
-        K = O.gram(spec, p.X)
-        w, _, _ = O.laplace_weight(K, p.y, lik, lp, tol=1e-5)
-        W, _ = O.laplace_precision(K, p.y, lik, lp, w)
-        O.predict(spec, p.X, p.Xtest, w, W)
-        done += 1
-        nsum += p.N
-        if time.perf_counter() - t0 > seconds:
-            break
-    dt = time.perf_counter() - t0
-    return dict(value=done / dt, unit="posteriors/s", cores=int(cores), kind="port",
-                sample=f"first {done} requests of the same stream (mean N {nsum / max(done, 1):.0f}), "
-                       f"reference-faithful LU Newton + dense predict solve, NumPy FP64, "
-                       f"{dt:.1f} s")
+    kind = O.EQ if p.kind == "EQ" else O.MATERN12
+    spec = O.KernelSpec.make(kind, p.variance, p.lengthscale, p.D)
+    lik = O.GAUSSIAN if p.lik == "gaussian" else O.PROBIT
+    lp = p.likelihood_parameters()
+    K = O.gram(spec, p.X)
+    w, _, _ = O.laplace_weight(K, p.y, lik, lp, tol=1e-5, form=variant)
+    W, _ = O.laplace_precision(K, p.y, lik, lp, w)
+    O.predict(spec, p.X, p.Xtest, w, W)
+    return time.perf_counter() - t0
+
+
+def _strata(problems, n):
+    """n requests standing for n equal-count strata of the given requests
+    ordered by N (the median of each stratum): their mean cost estimates the
+    mean cost per request of the whole set, whose cost grows like N^3."""
+    order = sorted(range(len(problems)), key=lambda i: problems[i].N)
+    n = max(1, min(n, len(order)))
+    edges = [round(k * len(order) / n) for k in range(n + 1)]
+    return [problems[order[(edges[k] + edges[k + 1] - 1) // 2]] for k in range(n)]
+
+
+def cpu_baseline(problems, strata, variant="lu", threads=None):
+    """The oracle on the host over a stratified sample of the timed requests
+    (one request per equal-count N-stratum): posteriors/s = 1 / mean seconds."""
+    try:
+        from threadpoolctl import threadpool_info, threadpool_limits
+    except Exception:  # pragma: no cover
+        threadpool_info = threadpool_limits = None
+    ctx = threadpool_limits(limits=threads) if (threadpool_limits and threads) else None
+    try:
+        if ctx is not None:
+            ctx.__enter__()
+        cores = (max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+                 if threadpool_info else (os.cpu_count() or 1))
+        sample = _strata(problems, strata)
+        t = [_cpu_request_seconds(p, variant) for p in sample]
+    finally:
+        if ctx is not None:
+            ctx.__exit__(None, None, None)
+    return sample, t, int(cores)
+
+
+def cpu_study(problems, strata, repeats):
+    """Both oracle variants, all host cores and one, median of `repeats` runs
+    of each stratified sample (the CPU-baseline plan); one JSON line."""
+    import platform
+    model = platform.processor()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    out = {"host_cpu": model, "nproc": os.cpu_count(), "strata": strata, "repeats": repeats,
+           "runs": {}}
+    for variant in ("lu", "bform"):
+        for threads in (None, 1):
+            rates = []
+            for _ in range(repeats):
+                sample, t, cores = cpu_baseline(problems, strata, variant, threads)
+                rates.append(len(t) / sum(t))
+            key = f"{variant}_{'all' if threads is None else 1}_cores"
+            out["runs"][key] = {"cores": cores, "posteriors_per_s_median": float(np.median(rates)),
+                                "runs": [round(r, 4) for r in rates]}
+    out["sample_mean_N"] = float(np.mean([p.N for p in _strata(problems, strata)]))
+    return out
+
+
+def _mix(problems):
+    n = np.array([p.N for p in problems], dtype=np.float64)
+    return {"requests": len(problems), "mean_N": round(float(n.mean()), 1),
+            "cbrt_mean_N3": round(float(np.mean(n ** 3) ** (1 / 3)), 1)}
+
+
+TIMER_CLASSES = (("outer_update", "TIMER_SYRK", "mfma"), ("column_steps", "TIMER_PANEL", "mfma"),
+                 ("trsv", "TIMER_TRSV", "hbm"), ("gram", "TIMER_GRAM", "hbm"),
+                 ("trsm", "TIMER_TRSM", "mfma"))
+
+
+def read_timer(lib, _lib, cls):
+    import ctypes as C
+    n_l, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+    _lib.check(lib.pgp_timer_read(cls, C.byref(n_l), C.byref(ms), C.byref(fl), C.byref(by)),
+               "pgp_timer_read")
+    return n_l.value, ms.value, fl.value, by.value
 
 
 def main():
@@ -198,9 +277,19 @@ def main():
               "serving path shards by routing requests to independent per-GPU engines, "
               "which is not run here", file=sys.stderr)
         sys.exit(2)
+    from probit_amd import data
+    R, K, W = args.requests_per_step, args.steps, args.warmup
+    total = (K + W) * R
+    if args.cpu_study:
+        stream = data.serving_stream(seed=args.seed, n=min(args.stream, total))
+        timed = [stream[i % len(stream)].problem for i in range(W * R, total)]
+        out = cpu_study(timed, args.cpu_strata, args.cpu_study)
+        out["timed_set"] = _mix(timed)
+        print(json.dumps({"cpu_study": out}))
+        return
     import torch
 
-    from probit_amd import _lib, data
+    from probit_amd import _lib
     from probit_amd.serving import ServingEngine
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     if args.copy_calibration:
@@ -215,8 +304,6 @@ def main():
         return
     lib = _lib.lib()
 
-    R, K, W = args.requests_per_step, args.steps, args.warmup
-    total = (K + W) * R
     stream = data.serving_stream(seed=args.seed, n=min(args.stream, total))
     if args.min_n > 0:
         stream = [r for r in data.serving_stream(seed=args.seed, n=args.stream)
@@ -225,9 +312,10 @@ def main():
             sys.exit(f"bench.py: no request of the stream has N >= {args.min_n}")
     order = [stream[i % len(stream)] for i in range(total)]
     dev_reqs = upload(order, torch)
-    # distinct request ids even when the stream is cycled
+    # distinct request ids even when the stream is cycled; sid = stream index
     for i, r in enumerate(dev_reqs):
         r["rid"] = i
+        r["sid"] = i % len(stream)
     torch.cuda.synchronize()
 
     eng = ServingEngine(precision=args.precision, capacity=args.capacity,
@@ -238,47 +326,105 @@ def main():
     torch.cuda.synchronize()
     lib.pgp_timer_reset()
     lib.pgp_timer_enable(1 << _lib.TIMER_SYRK)
+    timed = dev_reqs[W * R:(W + K) * R]
     results = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    results.update(serve_window(eng, dev_reqs[W * R:(W + K) * R]))
+    results.update(serve_window(eng, timed))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     lib.pgp_timer_enable(0)
-    import ctypes as C
-    n_l, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
-    _lib.check(lib.pgp_timer_read(_lib.TIMER_SYRK, C.byref(n_l), C.byref(ms), C.byref(fl),
-                                  C.byref(by)), "timer")
+    n_l, ms, fl, by = read_timer(lib, _lib, _lib.TIMER_SYRK)
     elapsed = t1 - t0
     value = K * R / elapsed
     lat_closed = np.array([v[0] for v in results.values()])
     iters = np.array([v[1].iterations for v in results.values()])
 
-    # plain FP64 per-request path on a sample of the timed requests
-    sample = dev_reqs[W * R:W * R + args.plain_sample]
-    plain, plain_rate, plain_lat = plain_fp64(sample, torch, _lib)
+    # per-class launch times over a window of the timed requests (every timer
+    # class on: the events add host work, so this window is not the timed one)
+    shares = None
+    if args.profile_window > 0:
+        stride = max(1, len(timed) // args.profile_window)
+        win = timed[::stride][:args.profile_window]
+        for r in win:
+            r["rid"] += 10 ** 7
+        lib.pgp_timer_reset()
+        mask = 0
+        for _, attr, _ in TIMER_CLASSES:
+            mask |= 1 << getattr(_lib, attr)
+        torch.cuda.synchronize()
+        lib.pgp_timer_enable(mask)
+        tp0 = time.perf_counter()
+        serve_window(eng, win)
+        torch.cuda.synchronize()
+        tp1 = time.perf_counter()
+        lib.pgp_timer_enable(0)
+        for r in win:
+            r["rid"] -= 10 ** 7
+        shares = {"requests": len(win), "wall_ms": round(1e3 * (tp1 - tp0), 3), "classes": {}}
+        for name, attr, bound in TIMER_CLASSES:
+            n, t_ms, f_, b_ = read_timer(lib, _lib, getattr(_lib, attr))
+            shares["classes"][name] = {
+                "launches": n, "total_ms": round(t_ms, 3),
+                "avg_launch_us": round(1e3 * t_ms / max(1, n), 3),
+                "share_of_wall": round(t_ms / max(1e-9, 1e3 * (tp1 - tp0)), 4),
+                "tflops": round(f_ / max(1e-12, t_ms * 1e-3) / 1e12, 3) if f_ else None,
+                "gbps": round(b_ / max(1e-12, t_ms * 1e-3) / 1e9, 3) if b_ else None,
+            }
+
+    # plain FP64 per-request path on the same requests: every distinct
+    # request of the timed set, its time weighted by how often the set holds it
+    sids = [r["sid"] for r in timed]
+    mult = {}
+    for sid in sids:
+        mult[sid] = mult.get(sid, 0) + 1
+    first = {}
+    for r in timed:
+        first.setdefault(r["sid"], r)
+    distinct = [first[k] for k in sorted(first)]
+    if args.plain_sample > 0:
+        step = max(1, len(distinct) // args.plain_sample)
+        distinct = distinct[::step][:args.plain_sample]
+    plain, secs = plain_fp64(distinct, torch, _lib)
+    w_tot = sum(mult[r["sid"]] for r in distinct)
+    plain_rate = w_tot / sum(mult[r["sid"]] * secs[r["sid"]] for r in distinct)
+    plain_lat = np.repeat([secs[r["sid"]] for r in distinct], [mult[r["sid"]] for r in distinct])
     errs = []
-    for r in sample:
-        c = results[r["rid"]][1]
-        m_ref = plain[r["rid"]][0].cpu().numpy()
+    by_sid = {}
+    for rid, (_, c) in results.items():
+        by_sid.setdefault(dev_reqs[rid]["sid"], c)
+    for r in distinct:
+        c = by_sid[r["sid"]]
+        m_ref = plain[r["sid"]][0].cpu().numpy()
         m = c.mean.cpu().numpy()
         errs.append(float(np.max(np.abs(m - m_ref)) / max(1e-300, np.max(np.abs(m_ref)))))
     errs = np.array(errs)
 
-    # open-loop latency at a fixed arrival rate
+    # open-loop latency at a fixed arrival rate: the stream's requests with
+    # Poisson arrivals, latency from arrival to completion
     rate = args.rate if args.rate > 0 else 0.7 * value
-    ol = dev_reqs[W * R:W * R + min(args.open_loop, K * R)]
+    ol = timed[:min(args.open_loop, len(timed))]
+    for r in ol:
+        r["rid"] += 2 * 10 ** 7
     t_base = time.perf_counter() + 0.01
     gaps = np.random.default_rng(args.seed + 1).exponential(1.0 / rate, len(ol))
     arrive = {r["rid"]: t_base + float(t) for r, t in zip(ol, np.cumsum(gaps))}
     open_res = serve_window(eng, ol, t_arrive=arrive)
+    for r in ol:
+        r["rid"] -= 2 * 10 ** 7
     lat_open = np.array([v[0] for v in open_res.values()])
 
-    cpu = cpu_baseline(order[W * R:], args.cpu_seconds)
+    timed_problems = [r["p"] for r in timed]
+    sample, t_cpu, cores = cpu_baseline(timed_problems, args.cpu_strata)
+    cpu = dict(value=len(t_cpu) / sum(t_cpu), unit="posteriors/s", cores=cores, kind="port",
+               sample=f"{len(sample)} requests, one per equal-count N-stratum of the timed set "
+                      f"(N {min(p.N for p in sample)}..{max(p.N for p in sample)}); "
+                      f"reference-faithful LU Newton + dense predict solve (the oracle), NumPy "
+                      f"FP64, {sum(t_cpu):.1f} s; posteriors/s = requests / total seconds")
 
     superstep = int(os.environ.get("PGP_SUPERSTEP", "8"))
-    ms_per_launch = ms.value / max(1, n_l.value)
-    achieved = fl.value / (ms.value * 1e-3) / 1e12 if ms.value > 0 else 0.0
+    ms_per_launch = ms / max(1, n_l)
+    achieved = fl / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
     # HBM bytes per launch of the outer update: the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated) give
@@ -286,7 +432,7 @@ def main():
     # (profiles/syrk_outer_traffic.json); times this run's algorithmic bytes
     # per launch, since the launch mix varies with the serving dynamics.
     # null if not measured
-    algo_bytes = by.value / max(1, n_l.value)
+    algo_bytes = by / max(1, n_l)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "syrk_outer_traffic.json")
     if os.path.exists(tpath):
@@ -317,19 +463,28 @@ def main():
             "min_n": args.min_n,
             "precision": args.precision,
             "tolerance": 1e-5,
+            "timed_set": _mix(timed_problems),
         },
-        "p50_latency_ms_closed_loop": round(1e3 * float(np.percentile(lat_closed, 50)), 3),
-        "p99_latency_ms_closed_loop": round(1e3 * float(np.percentile(lat_closed, 99)), 3),
+        "p50_latency_ms": round(1e3 * float(np.percentile(lat_open, 50)), 3),
+        "p99_latency_ms": round(1e3 * float(np.percentile(lat_open, 99)), 3),
         "open_loop": {
             "rate_req_per_s": round(rate, 1),
             "requests": len(lat_open),
             "p50_latency_ms": round(1e3 * float(np.percentile(lat_open, 50)), 3),
             "p99_latency_ms": round(1e3 * float(np.percentile(lat_open, 99)), 3),
+            "latency_from": "arrival",
+        },
+        "closed_loop": {
+            "p50_latency_ms": round(1e3 * float(np.percentile(lat_closed, 50)), 3),
+            "p99_latency_ms": round(1e3 * float(np.percentile(lat_closed, 99)), 3),
+            "latency_from": "admission",
         },
         "plain_fp64": {
             "value": round(plain_rate, 2),
             "unit": "posteriors/s",
-            "requests": len(sample),
+            "requests": "every distinct request of the timed set, weighted by its multiplicity"
+                        if args.plain_sample <= 0 else f"strided sample of {len(distinct)}",
+            "distinct_requests": len(distinct),
             "p50_latency_ms": round(1e3 * float(np.percentile(plain_lat, 50)), 3),
             "p99_latency_ms": round(1e3 * float(np.percentile(plain_lat, 99)), 3),
         },
@@ -345,11 +500,12 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 5),
             "traffic": traffic,
-            "launches": int(n_l.value),
+            "launches": int(n_l),
             "avg_launch_us": round(1e3 * ms_per_launch, 3),
-            "algo_flops_per_launch": fl.value / max(1, n_l.value),
+            "algo_flops_per_launch": fl / max(1, n_l),
             "algo_bytes_per_launch": algo_bytes,
         },
+        "kernel_classes": shares,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

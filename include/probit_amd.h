@@ -126,6 +126,14 @@ typedef struct pgp_problem {
 const char* pgp_last_error(void);
 int pgp_version(void);
 
+/* The mapping every fit entry point applies to a problem's device status
+ * word (int32 at the workspace header): PGP_INFO_NOT_PD / PGP_INFO_STALLED
+ * are ORed into result->status, and result->objective becomes NaN where
+ * NOT_PD, STALLED or NEGATIVE_W mark the numbers invalid (the reference's
+ * JAX path yields NaN there).  Host only; exported for callers that read
+ * status words themselves.  No counterpart in the reference. */
+int pgp_fit_result_apply_info(pgp_fit_result_t* result, int32_t info_word);
+
 /* Opt-in launch timer for measurement: with bit c of `mask` set, launches of
  * kernel class c (0 = outer trailing SYRK update of the Cholesky, 1 = its
  * per-column steps: diagonal factor, panel, column update, 2 = TRSV,
@@ -241,7 +249,12 @@ int pgp_laplace_set_state(const pgp_problem_t* problem,
 int64_t pgp_predict_workspace_size(int64_t N, int64_t M, int32_t precision);
 
 /* Approximator.predict (probit/approximators.py:154-180) from the resident
- * factor: mean = Kfs^T a, var = Kss - ||L_B^-1 W^1/2 Kfs||^2 (latent). */
+ * factor: mean = Kfs^T a, var = Kss - ||L_B^-1 W^1/2 Kfs||^2 (latent).
+ * `kernel` must be the resident fit's kernel (the reference's predict takes
+ * the parameters it uses): other parameters, or a workspace that holds no
+ * predict state (nothing fitted, the VB map's sigma^2 I + K), give a
+ * negative status.  Waits for its work; returns PGP_INFO_NOT_PD /
+ * PGP_INFO_STALLED (positive) if the factor or the solve is invalid. */
 int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M,
                 double* mean, double* var, void* workspace,
                 int64_t workspace_bytes, void* predict_workspace,
@@ -252,7 +265,8 @@ int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M,
  * cov = Kss - Kfs^T (K + W^-1)^-1 Kfs, column-major M x M FP64 with leading
  * dimension ld >= M, computed as Kss - V^T V with V = L_B^-1 W^1/2 Kfs (the
  * same triangular solve as pgp_predict; FP64 MFMA product) and written
- * exactly symmetric.  Workspaces as for pgp_predict. */
+ * exactly symmetric.  Workspaces, the kernel check and the status as for
+ * pgp_predict. */
 int pgp_predict_covariance(const pgp_kernel_t* kernel, const double* Xtest,
                            int64_t M, double* cov, int64_t ld, void* workspace,
                            int64_t workspace_bytes, void* predict_workspace,
@@ -266,7 +280,10 @@ int64_t pgp_grad_workspace_size(int64_t N, int32_t D);
  * jax.value_and_grad yields through fixed_point_layer
  * (probit/approximators.py:132-134, probit/implicit/solvers.py:52-64).
  * grad_out is HOST memory of 2 + 2*D + PGP_MAX_CUT doubles laid out as
- * [variance, lengthscale[D], period[D], noise_std, cutpoints[J+1]]. */
+ * [variance, lengthscale[D], period[D], noise_std, cutpoints[J+1]].
+ * `kernel` and `lik` must be the resident Laplace fit's (else a negative
+ * status); a non-positive pivot or an expired wait gives NaN gradients and
+ * returns PGP_INFO_NOT_PD / PGP_INFO_STALLED. */
 int pgp_laplace_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik,
                      double* grad_out, void* workspace, int64_t workspace_bytes,
                      void* grad_workspace, int64_t grad_workspace_bytes,
@@ -291,7 +308,8 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
  * (probit/approximators.py:132-134,316-330, probit/implicit/solvers.py:52-64,
  * probit/implicit/VB.py:4-40).  A likelihood curvature below zero somewhere
  * (the probit floor) gives NaN.  The resident state (the factor predict
- * uses) is the same afterwards. */
+ * uses) is the same afterwards.  Argument checks and status as for
+ * pgp_laplace_grad, against the resident VB fit. */
 /* The VB fixed-point map f_VB(w) = (sigma^2 I + K)^-1 (K w + sigma
  * d log p(y | K w) / df) into problem->weight (probit/implicit/VB.py:4-16,
  * VBGP.construct, probit/approximators.py:306-314).  Leaves sigma^2 I + K
@@ -335,10 +353,11 @@ typedef struct pgp_batch_item {
  * step shared by all of them, then finalises and predicts the items that
  * converged and returns their tags.  Items join and leave between steps.
  * An item's results do not depend on which other items share its steps.
- * The finalisation and predict of converged items run on a second stream
- * that the batcher creates (default priority) and joins to `stream` within
- * each step, so work the caller enqueues on `stream` afterwards sees the
- * whole step.  This is the batched form of LaplaceGP.approximate_posterior + predict
+ * The steps run on streams the batcher creates (default priority); each
+ * step first waits for what the caller had enqueued on `stream` when it was
+ * started (the items' inputs), and step() returns (finish() reports) only
+ * after the step's work is complete, so the results are ready for any
+ * later work.  This is the batched form of LaplaceGP.approximate_posterior + predict
  * (probit/approximators.py:204-210, 154-180). */
 typedef struct pgp_batcher pgp_batcher_t;
 int64_t pgp_batcher_table_size(int32_t capacity);

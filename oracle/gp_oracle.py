@@ -76,7 +76,16 @@ def gram(spec: KernelSpec, X: np.ndarray, Z: np.ndarray | None = None) -> np.nda
     (probit/approximators.py:173).
     """
     Z = X if Z is None else Z
-    r2 = np.sum(_q(spec, X, Z) / spec.lengthscales**2, axis=-1)
+    # r^2 accumulated one input dimension at a time (N x M memory, not N x M x D)
+    r2 = np.zeros((X.shape[0], Z.shape[0]))
+    for d in range(X.shape[1]):
+        diff = X[:, d, None] - Z[None, :, d]
+        if spec.periods is None:
+            q = diff * diff
+        else:
+            sn = np.sin(np.pi * diff / spec.periods[d])
+            q = 4.0 * sn * sn
+        r2 += q / spec.lengthscales[d] ** 2
     if spec.kind == EQ:
         return spec.variance * np.exp(-0.5 * r2)
     return spec.variance * np.exp(-np.sqrt(np.maximum(r2, 0.0)))

@@ -124,6 +124,7 @@ _I32 = C.c_int32
 _SIGNATURES = {
     "pgp_last_error": (C.c_char_p, []),
     "pgp_version": (C.c_int, []),
+    "pgp_fit_result_apply_info": (C.c_int, [C.POINTER(FitResultT), C.c_int32]),
     "pgp_timer_enable": (C.c_int, [C.c_uint32]),
     "pgp_timer_reset": (C.c_int, []),
     "pgp_timer_read": (C.c_int, [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
@@ -206,6 +207,15 @@ def check(rc: int, what: str) -> None:
     if rc != 0:
         msg = load().pgp_last_error().decode(errors="replace")
         raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+
+
+def info_bits(rc: int, what: str) -> int:
+    """Entry points that act on a resident factor return a negative status
+    for a bad call (raised here) and positive INFO_* bits when their numbers
+    are invalid (returned)."""
+    if rc < 0:
+        check(rc, what)
+    return int(rc)
 
 
 def stream_handle(device=None) -> int:

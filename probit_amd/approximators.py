@@ -134,7 +134,9 @@ class Approximator(ABC):
         self._pws = None
         self._gws = None
         self._state = None
+        self._generation = 0
         self.last_result = None
+        self.last_status = 0
 
     # ---- packing ----------------------------------------------------------
     def _kernel(self, prior_parameters) -> Kernel:
@@ -201,11 +203,23 @@ class Approximator(ABC):
         if msg:
             raise FloatingPointError(f"{what}: {msg} (status {res.status})")
 
+    def _check_bits(self, bits, what):
+        """Status bits a predict or gradient call returned: strict=True raises,
+        otherwise the outputs are NaN (the reference's behaviour)."""
+        self.last_status = bits
+        if bits and self.strict:
+            raise FloatingPointError(f"{what}: {numerical_failure(bits)} (status {bits})")
+
     # ---- resident-state bookkeeping -----------------------------------------
-    def _remember(self, key, weight, precision, result):
+    def _remember(self, key, weight, precision, result, fixed_point=False):
+        """Record what the workspace holds.  Every state gets a new generation;
+        `fixed_point` marks one formed at the fixed point of `key` (a fit, or
+        a batched fit's weight), where the adjoint is valid."""
+        self._generation = getattr(self, "_generation", 0) + 1
         self._state = dict(key=key, weight=weight, precision=precision,
                            wv=(weight.data_ptr(), weight._version),
-                           pv=(precision.data_ptr(), precision._version))
+                           pv=(precision.data_ptr(), precision._version),
+                           gen=self._generation, fixed_point=bool(fixed_point))
         self.last_result = result
 
     def _state_matches(self, key, weight, precision) -> bool:
@@ -221,7 +235,7 @@ class Approximator(ABC):
         t = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))
         return t.detach().to(torch.float64).reshape(-1).to(self.device).contiguous()
 
-    def _set_state(self, ks, ls, weight, precision):
+    def _set_state(self, ks, ls, weight, precision, fixed_point=False):
         """Factor B at a caller-given (weight, precision) (predict's input)."""
         lib = _lib.lib()
         ws = self._workspace()
@@ -235,7 +249,7 @@ class Approximator(ABC):
                                        self._stream())
         _lib.check(rc, "pgp_laplace_set_state")
         Wt = W if W is not None else Wout
-        self._remember(self._key(ks, ls), w, Wt, res)
+        self._remember(self._key(ks, ls), w, Wt, res, fixed_point)
         self._check_status(res, "pgp_laplace_set_state")
         return w, Wt, fout, res
 
@@ -273,7 +287,11 @@ class Approximator(ABC):
         rc = lib.pgp_predict(ks, Xt.data_ptr(), M, mean.data_ptr(), var.data_ptr(),
                              ws.data_ptr(), ws.numel(), self._pws.data_ptr(), self._pws.numel(),
                              self._stream())
-        _lib.check(rc, "pgp_predict")
+        bits = _lib.info_bits(rc, "pgp_predict")
+        if bits:
+            mean.fill_(float("nan"))
+            var.fill_(float("nan"))
+        self._check_bits(bits, "pgp_predict")
         return mean, var
 
     def predict_covariance(self, X_test, parameters, weight, precision):
@@ -287,7 +305,10 @@ class Approximator(ABC):
         rc = lib.pgp_predict_covariance(ks, Xt.data_ptr(), M, cov.data_ptr(), M, ws.data_ptr(),
                                         ws.numel(), self._pws.data_ptr(), self._pws.numel(),
                                         self._stream())
-        _lib.check(rc, "pgp_predict_covariance")
+        bits = _lib.info_bits(rc, "pgp_predict_covariance")
+        if bits:
+            cov.fill_(float("nan"))
+        self._check_bits(bits, "pgp_predict_covariance")
         return cov
 
     def posterior_mean(self, weight):
@@ -348,10 +369,11 @@ class _LaplaceEvidence(torch.autograd.Function):
         ks, lsd = gp._structs(parameters)
         pre = getattr(gp, "_prefit", {}).get(gp._key(ks, lsd))
         if pre is not None:  # fixed point from a batched fit: re-form its state
-            res = gp._set_state(ks, lsd, pre, None)[3]
+            res = gp._set_state(ks, lsd, pre, None, fixed_point=True)[3]
         else:
             res = gp._run_fit(ks, lsd)[2]
         ctx.gp = gp
+        ctx.gen = gp._state["gen"]  # the fixed-point state the adjoint must use
         ctx.key = gp._key(ks, lsd)
         ctx.structs = (ks, lsd)
         ctx.parameters = parameters
@@ -365,7 +387,11 @@ class _LaplaceEvidence(torch.autograd.Function):
     def backward(ctx, g):
         gp = ctx.gp
         ks, lsd = ctx.structs
-        if gp._state is None or gp._state["key"] != ctx.key:
+        st = gp._state
+        # anything that re-formed the workspace since forward (predict,
+        # precision or construct with other values, another fit) leaves a state
+        # that is not this fixed point: fit again before the adjoint
+        if st is None or st["gen"] != ctx.gen or st["key"] != ctx.key or not st["fixed_point"]:
             gp._run_fit(ks, lsd)
         vec = gp._grad(ks, lsd)
         D = ctx.D
@@ -480,7 +506,7 @@ class LaplaceGP(Approximator):
         rc = lib.pgp_laplace_fit(self._problem(weight, precision), ks, ls, self._options(), res,
                                  ws.data_ptr(), ws.numel(), self._stream())
         _lib.check(rc, "pgp_laplace_fit")
-        self._remember(self._key(ks, ls), weight, precision, res)
+        self._remember(self._key(ks, ls), weight, precision, res, fixed_point=True)
         self._check_status(res, "pgp_laplace_fit")
         if res.status & _lib.INFO_MAXITER:
             warnings.warn(f"Newton iteration stopped at maxiter={self.maxiter} "
@@ -499,7 +525,7 @@ class LaplaceGP(Approximator):
         out = (np.zeros(2 + 2 * self.D + _lib.MAX_CUT)).astype(np.float64)
         rc = lib.pgp_laplace_grad(ks, ls, out.ctypes.data, ws.data_ptr(), ws.numel(),
                                   gws.data_ptr(), gws.numel(), self._stream())
-        _lib.check(rc, "pgp_laplace_grad")
+        self._check_bits(_lib.info_bits(rc, "pgp_laplace_grad"), "pgp_laplace_grad")
         return out
 
     def construct(self):
@@ -569,7 +595,7 @@ class VBGP(Approximator):
                             ws.data_ptr(), ws.numel(), gws.data_ptr(), gws.numel(),
                             self._stream())
         _lib.check(rc, "pgp_vb_fit")
-        self._remember(self._key(ks, ls), weight, precision, res)
+        self._remember(self._key(ks, ls), weight, precision, res, fixed_point=True)
         self._check_status(res, "pgp_vb_fit")
         if res.status & _lib.INFO_MAXITER:
             warnings.warn(f"fixed-point iteration stopped at maxiter={self.maxiter}")
@@ -606,7 +632,7 @@ class VBGP(Approximator):
         out = (np.zeros(2 + 2 * self.D + _lib.MAX_CUT)).astype(np.float64)
         rc = lib.pgp_vb_grad(ks, ls, out.ctypes.data, ws.data_ptr(), ws.numel(),
                              gws.data_ptr(), gws.numel(), self._stream())
-        _lib.check(rc, "pgp_vb_grad")
+        self._check_bits(_lib.info_bits(rc, "pgp_vb_grad"), "pgp_vb_grad")
         return out
 
     def objective(self):

@@ -299,6 +299,58 @@ static int read_prob(const char* ws, Prob& p, hipStream_t s) {
   return 0;
 }
 
+// Entry points that act on the resident factor (predict, covariance,
+// gradients) compute from the resident descriptor, so the kernel (and, for
+// the gradients, likelihood) a caller passes must be the resident fit's:
+// the reference's predict(X_test, parameters, ...) and value_and_grad()(parameters)
+// use the parameters they are given (probit/approximators.py:171-174, :132-134).
+// `states` is the set of ResidentState values (bit per value) the call accepts.
+static int check_resident(const Prob& p, unsigned states, const char* who,
+                          const pgp_kernel_t* k, int karg, const pgp_likelihood_t* l,
+                          int larg) {
+  if (!((states >> p.state) & 1u)) {
+    static const char* names[] = {"nothing fitted", "a Laplace state", "a VB fit",
+                                  "the VB map's sigma^2 I + K"};
+    return fail(-99, "%s: the workspace holds %s", who,
+                (p.state >= 0 && p.state <= RS_VBMAP) ? names[p.state] : "an unknown state");
+  }
+  if (!k) return fail(karg, "%s: kernel is NULL", who);
+  if (k->dim != p.D) return fail(karg, "%s: kernel dim %d != the resident fit's D %d", who, k->dim, p.D);
+  Prob q;
+  memset(&q, 0, sizeof q);
+  fill_kernel(q, k);
+  bool same = q.kern_kind == p.kern_kind && q.variance == p.variance;
+  for (int d = 0; d < p.D; ++d) same = same && q.ls[d] == p.ls[d] && q.per[d] == p.per[d];
+  if (!same)
+    return fail(karg, "%s: kernel parameters differ from the resident fit's (fit again, or "
+                      "re-form the state with pgp_laplace_set_state, first)", who);
+  if (larg) {
+    if (!l) return fail(larg, "%s: likelihood is NULL", who);
+    fill_lik(q, l);
+    same = q.lik_kind == p.lik_kind && q.noise_std == p.noise_std;
+    if (same && q.lik_kind != PGP_LIK_GAUSSIAN) {
+      same = q.J == p.J && (q.lik_kind != PGP_LIK_PROBIT_CLOSED || q.sp_bounds == p.sp_bounds);
+      for (int c = 0; same && c <= q.J; ++c) same = q.cut[c] == p.cut[c];
+    }
+    if (!same)
+      return fail(larg, "%s: likelihood parameters differ from the resident fit's (fit again "
+                        "first)", who);
+  }
+  return 0;
+}
+
+constexpr unsigned RS_PREDICT = (1u << RS_LAPLACE) | (1u << RS_VB);
+
+// Status bits of the factor and of the solves since the state was formed
+// (a non-positive pivot, an expired bounded wait): after the call's work.
+static int read_info_bits(const char* ws, hipStream_t s, int* bits) {
+  int32_t info = 0;
+  HIP_OK(hipMemcpyAsync(&info, ws + OFF_INFO, sizeof info, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  *bits = info & (PGP_INFO_NOT_PD | PGP_INFO_STALLED);
+  return 0;
+}
+
 static Batch single_batch(char* ws, int Np) {
   Batch b;
   b.dprobs = (Prob* const*)(ws + OFF_ARR);
@@ -365,15 +417,32 @@ static void laplace_finalize(const Batch& all, const Batch* fac, const Batch* fa
   launch_vec_op(all, OP_OUT, s);
 }
 
+// Status bits of a fit from its device status word, and a NaN objective
+// where the bits make the numbers invalid (as JAX would give NaN).
+static void apply_info(pgp_fit_result_t* r, int info) {
+  if (info & PGP_INFO_NOT_PD) r->status |= PGP_INFO_NOT_PD;
+  if (info & PGP_INFO_STALLED) r->status |= PGP_INFO_STALLED;
+  if (r->status & (PGP_INFO_NEGATIVE_W | PGP_INFO_NOT_PD | PGP_INFO_STALLED)) r->objective = NAN;
+}
+
 static void fill_result_final(pgp_fit_result_t* r, const double* g4, int info) {
   r->sum_log_lik = g4[0];
   r->half_f_dot_a = 0.5 * g4[1];
   r->log_det = 2.0 * g4[2];
   if (g4[3] > 0) r->status |= PGP_INFO_NEGATIVE_W;
-  if (info & PGP_INFO_NOT_PD) r->status |= PGP_INFO_NOT_PD;
-  if (info & PGP_INFO_STALLED) r->status |= PGP_INFO_STALLED;
   r->objective = -g4[0] + 0.5 * g4[1] + g4[2];
-  if (r->status & (PGP_INFO_NEGATIVE_W | PGP_INFO_NOT_PD | PGP_INFO_STALLED)) r->objective = NAN;
+  apply_info(r, info);
+}
+
+// Clear only the non-positive-pivot bit of each member's status word (a
+// stall flagged in the same iteration must survive into the final status).
+__global__ void k_clear_not_pd(Prob* const* arr, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAnd(arr[i]->info, ~(int32_t)PGP_INFO_NOT_PD);
+}
+static void launch_clear_not_pd(const Batch& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_not_pd, dim3((b.count + 255) / 256), dim3(256), 0, s, b.dprobs,
+                     b.count);
 }
 
 // FP64 refinement steps per Newton solve by default.  The Newton residual is
@@ -472,7 +541,13 @@ int pgp_timer_read(int32_t cls, int64_t* launches, double* total_ms, double* alg
   if (algo_bytes) *algo_bytes = by;
   return 0;
 }
-int pgp_version(void) { return 1; }
+int pgp_version(void) { return 2; }
+
+int pgp_fit_result_apply_info(pgp_fit_result_t* result, int32_t info_word) {
+  if (!result) return fail(-1, "result is NULL");
+  apply_info(result, info_word);
+  return 0;
+}
 
 int pgp_gram(const pgp_kernel_t* kernel, int dtype, const double* X, int64_t N, const double* Z,
              int64_t M, void* out, int64_t ld, void* stream) {
@@ -604,6 +679,7 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   char* ws = (char*)workspace;
   const int refine = opts->refine >= 0 ? opts->refine : default_refine(prec);
   Prob p = make_prob(problem, kernel, lik, prec, ws);
+  p.state = RS_LAPLACE;
   write_prob(ws, p, 1, s);
   set_self(ws, s);
   const Batch b = single_batch(ws, p.Np);
@@ -636,7 +712,7 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
       cur = PGP_PREC_FP32;
       result->status |= PGP_INFO_ESCALATED;
       force_factor = true;
-      HIP_OK(hipMemsetAsync(ws + OFF_INFO, 0, sizeof(int32_t), s));
+      launch_clear_not_pd(b, s);
       continue;
     }
     if (cur == PGP_PREC_BF16 && stalled(it + 1, st, step)) {
@@ -680,6 +756,7 @@ int pgp_vb_map(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   Prob p = make_prob(problem, kernel, lik, precision, ws);
   p.out_prec = nullptr;
   p.out_latent = nullptr;
+  p.state = RS_VBMAP;
   write_prob(ws, p, 1, s);
   set_self(ws, s);
   const Batch b = single_batch(ws, p.Np);
@@ -710,6 +787,7 @@ int pgp_laplace_set_state(const pgp_problem_t* problem, const pgp_kernel_t* kern
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   Prob p = make_prob(problem, kernel, lik, precision, ws);
+  p.state = RS_LAPLACE;
   write_prob(ws, p, 1, s);
   set_self(ws, s);
   const Batch b = single_batch(ws, p.Np);
@@ -740,7 +818,7 @@ int pgp_laplace_set_state(const pgp_problem_t* problem, const pgp_kernel_t* kern
 int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M, double* mean,
                 double* var, void* workspace, int64_t workspace_bytes, void* predict_workspace,
                 int64_t predict_workspace_bytes, void* stream) {
-  (void)kernel;
+  if (!kernel) return fail(-1, "kernel is NULL");
   if (!Xtest || M <= 0) return fail(-2, "Xtest / M invalid");
   if (!mean || !var) return fail(-4, "mean / var are NULL");
   if (!workspace) return fail(-6, "workspace is NULL");
@@ -748,6 +826,8 @@ int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M, doub
   char* ws = (char*)workspace;
   Prob p;
   int rc = read_prob(ws, p, s);
+  if (rc) return rc;
+  rc = check_resident(p, RS_PREDICT, "pgp_predict", kernel, -1, nullptr, 0);
   if (rc) return rc;
   const WsLayout L = ws_layout(p.N, p.prec);
   if (workspace_bytes < L.total) return fail(-7, "workspace too small");
@@ -768,14 +848,18 @@ int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M, doub
   launch_cross_prep(b, p.prec, (int)M, s);
   launch_trsm(b, p.prec, (int)PL.Mp, 0, s);
   launch_predict_final(b, p.prec, (int)M, s);
-  return check_launch();
+  rc = check_launch();
+  if (rc) return rc;
+  int bits = 0;
+  rc = read_info_bits(ws, s, &bits);
+  return rc ? rc : bits;
 }
 
 int pgp_predict_covariance(const pgp_kernel_t* kernel, const double* Xtest, int64_t M,
                            double* cov, int64_t ld, void* workspace, int64_t workspace_bytes,
                            void* predict_workspace, int64_t predict_workspace_bytes,
                            void* stream) {
-  (void)kernel;
+  if (!kernel) return fail(-1, "kernel is NULL");
   if (!Xtest || M <= 0) return fail(-2, "Xtest / M invalid");
   if (!cov) return fail(-4, "cov is NULL");
   if (ld < M) return fail(-5, "ld < M");
@@ -784,6 +868,8 @@ int pgp_predict_covariance(const pgp_kernel_t* kernel, const double* Xtest, int6
   char* ws = (char*)workspace;
   Prob p;
   int rc = read_prob(ws, p, s);
+  if (rc) return rc;
+  rc = check_resident(p, RS_PREDICT, "pgp_predict_covariance", kernel, -1, nullptr, 0);
   if (rc) return rc;
   const WsLayout L = ws_layout(p.N, p.prec);
   if (workspace_bytes < L.total) return fail(-7, "workspace too small");
@@ -804,20 +890,26 @@ int pgp_predict_covariance(const pgp_kernel_t* kernel, const double* Xtest, int6
   launch_cross_prep(b, p.prec, (int)M, s);
   launch_trsm(b, p.prec, (int)PL.Mp, 0, s);
   launch_pred_cov(b, p.prec, (int)PL.Mp, cov, ld, s);
-  return check_launch();
+  rc = check_launch();
+  if (rc) return rc;
+  int bits = 0;
+  rc = read_info_bits(ws, s, &bits);
+  return rc ? rc : bits;
 }
 
 int pgp_laplace_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik, double* grad_out,
                      void* workspace, int64_t workspace_bytes, void* grad_workspace,
                      int64_t grad_workspace_bytes, void* stream) {
-  (void)kernel;
-  (void)lik;
+  if (!kernel) return fail(-1, "kernel is NULL");
+  if (!lik) return fail(-2, "likelihood is NULL");
   if (!grad_out) return fail(-3, "grad_out is NULL");
   if (!workspace) return fail(-4, "workspace is NULL");
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   Prob p;
   int rc = read_prob(ws, p, s);
+  if (rc) return rc;
+  rc = check_resident(p, 1u << RS_LAPLACE, "pgp_laplace_grad", kernel, -1, lik, -2);
   if (rc) return rc;
   if (p.D > grad_maxd()) return fail(-1, "gradient supports D <= %d (got %d)", grad_maxd(), p.D);
   if (p.lik_kind == PGP_LIK_PROBIT_CLOSED)
@@ -859,15 +951,19 @@ int pgp_laplace_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik, do
   std::vector<double> hl(1 + PGP_MAX_CUT), hk(grad_npar());
   HIP_OK(hipMemcpyAsync(hl.data(), lik_out, hl.size() * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(hipMemcpyAsync(hk.data(), kern_out, hk.size() * 8, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
+  int bits = 0;
+  rc = read_info_bits(ws, s, &bits);  // also waits for the copies
+  if (rc) return rc;
   const int D = p.D, gm = grad_maxd();
+  // an invalid factor or an expired wait in the adjoint's solves: NaN
+  const double bad = bits ? NAN : 0.0;
   int o = 0;
-  grad_out[o++] = hk[0];
-  for (int d = 0; d < D; ++d) grad_out[o++] = hk[1 + d];
-  for (int d = 0; d < D; ++d) grad_out[o++] = p.periodic ? hk[1 + gm + d] : 0.0;
-  grad_out[o++] = hl[0];
-  for (int c = 0; c < PGP_MAX_CUT; ++c) grad_out[o++] = hl[1 + c];
-  return 0;
+  grad_out[o++] = hk[0] + bad;
+  for (int d = 0; d < D; ++d) grad_out[o++] = hk[1 + d] + bad;
+  for (int d = 0; d < D; ++d) grad_out[o++] = (p.periodic ? hk[1 + gm + d] : 0.0) + bad;
+  grad_out[o++] = hl[0] + bad;
+  for (int c = 0; c < PGP_MAX_CUT; ++c) grad_out[o++] = hl[1 + c] + bad;
+  return bits;
 }
 
 // Gradient of objective_VB at the resident VB fixed point (after pgp_vb_fit).
@@ -888,14 +984,16 @@ int pgp_laplace_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik, do
 int pgp_vb_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik, double* grad_out,
                 void* workspace, int64_t workspace_bytes, void* grad_workspace,
                 int64_t grad_workspace_bytes, void* stream) {
-  (void)kernel;
-  (void)lik;
+  if (!kernel) return fail(-1, "kernel is NULL");
+  if (!lik) return fail(-2, "likelihood is NULL");
   if (!grad_out) return fail(-3, "grad_out is NULL");
   if (!workspace) return fail(-4, "workspace is NULL");
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   Prob p;
   int rc = read_prob(ws, p, s);
+  if (rc) return rc;
+  rc = check_resident(p, 1u << RS_VB, "pgp_vb_grad", kernel, -1, lik, -2);
   if (rc) return rc;
   if (p.D > grad_maxd()) return fail(-1, "gradient supports D <= %d (got %d)", grad_maxd(), p.D);
   if (p.lik_kind == PGP_LIK_PROBIT_CLOSED)
@@ -950,18 +1048,21 @@ int pgp_vb_grad(const pgp_kernel_t* kernel, const pgp_likelihood_t* lik, double*
   HIP_OK(hipMemcpyAsync(hk.data(), kern_out, hk.size() * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(hipMemcpyAsync(hs, sig_out, sizeof hs, hipMemcpyDeviceToHost, s));
   HIP_OK(hipMemcpyAsync(g4, gather, sizeof g4, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
+  int bits = 0;
+  rc = read_info_bits(ws, s, &bits);  // also waits for the copies
+  if (rc) return rc;
   const int D = p.D, gm = grad_maxd();
   const double sigma = p.noise_std;
-  // W < 0 somewhere (the probit floor): the Woodbury form does not hold
-  const double bad = g4[3] > 0 ? NAN : 0.0;
+  // W < 0 somewhere (the probit floor): the Woodbury form does not hold;
+  // an invalid factor or an expired wait in the adjoint's solves: NaN too
+  const double bad = (g4[3] > 0 || bits) ? NAN : 0.0;
   int o = 0;
   grad_out[o++] = hk[0] + bad;
   for (int d = 0; d < D; ++d) grad_out[o++] = hk[1 + d] + bad;
   for (int d = 0; d < D; ++d) grad_out[o++] = (p.periodic ? hk[1 + gm + d] : 0.0) + bad;
   grad_out[o++] = hl[0] - p.N / sigma + sigma * hs[0] + hs[1] + bad;
   for (int c = 0; c < PGP_MAX_CUT; ++c) grad_out[o++] = hl[1 + c] + bad;
-  return 0;
+  return bits;
 }
 
 int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
@@ -987,6 +1088,7 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   p.C = (double*)(gw + G.C);
   p.S = gw + G.G2;
   p.Mp = p.Np;
+  p.state = RS_VB;
   write_prob(ws, p, 1, s);
   set_self(ws, s);
   const Batch b = single_batch(ws, p.Np);
@@ -1053,10 +1155,10 @@ int pgp_vb_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   result->sum_log_lik = g4[0];
   result->half_f_dot_a = 0.5 * g4[1];
   result->log_det = 2.0 * g4[2];  // log |sigma^2 I + K|
-  if (info[0] & PGP_INFO_NOT_PD) result->status |= PGP_INFO_NOT_PD;
   // objective_VB (probit/implicit/VB.py:33-39)
   result->objective = 0.5 * tr[1] + 0.5 * sigma * sigma * tr[0] + 0.5 * g4[1] -
                       N * log(sigma) + g4[2] - 0.5 * N - g4[0];
+  apply_info(result, info[0]);  // NaN objective on a non-positive pivot or an expired wait
   return 0;
 }
 
@@ -1107,7 +1209,7 @@ struct BItem {
 };
 
 struct Lane {
-  hipStream_t s = nullptr;   // Newton steps (lane 0: the caller's stream)
+  hipStream_t s = nullptr;   // Newton steps (the lane's own stream)
   hipStream_t s2 = nullptr;  // finalisation and predict, beside the Newton step
   bool own_s = false;
   hipEvent_t ev = nullptr, ev_fork = nullptr, ev_join = nullptr;
@@ -1131,6 +1233,9 @@ struct pgp_batcher {
   pgp_fit_options_t opts;
   int refine;
   int cap;
+  hipStream_t caller = nullptr;     // the stream given at creation
+  hipEvent_t ev_caller = nullptr;   // its state at each launch / step
+
   int lane_np[pgp::NLANES - 1];  // size class bounds on Np (0: unused)
   pgp::Lane lane[pgp::NLANES];
   std::list<pgp::BItem> items;  // stable addresses: lanes hold pointers
@@ -1151,15 +1256,16 @@ static bool on_s2(int which) { return which == 3 || which == 4 || which == 5 || 
 // workspaces with one staged copy and one kernel.  `region` (0 or 1) picks
 // the half of the lane's staging buffers, so that two writes of one step do
 // not overlap; `arr` is the batch's device pointer array (already written).
-static void write_probs(pgp_batcher* bt, Lane& L, const std::vector<BItem*>& v,
-                        Prob* const* arr, int region, int reset, hipStream_t s) {
-  if (v.empty()) return;
+static int write_probs(pgp_batcher* bt, Lane& L, const std::vector<BItem*>& v,
+                       Prob* const* arr, int region, int reset, hipStream_t s) {
+  if (v.empty()) return 0;
   Prob* h = L.hdesc + (size_t)region * bt->cap;
   Prob* d = L.ddesc + (size_t)region * bt->cap;
   for (size_t i = 0; i < v.size(); ++i) h[i] = v[i]->p;
-  (void)hipMemcpyAsync(d, h, v.size() * sizeof(Prob), hipMemcpyHostToDevice, s);
+  HIP_OK(hipMemcpyAsync(d, h, v.size() * sizeof(Prob), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_scatter_probs, dim3((unsigned)v.size()), dim3(256), 0, s, arr, d,
                      (int)v.size(), reset);
+  return check_launch();
 }
 
 // Members are ordered by padded size, largest first (stable), so that the
@@ -1206,19 +1312,31 @@ static void admit(pgp_batcher* bt) {
 
 // New members of `fresh` get their descriptor, Gram matrix and a zero
 // weight (on the lane's Newton stream).
-static void start_members(pgp_batcher* bt, Lane& L, std::vector<BItem*>& fresh) {
-  if (fresh.empty()) return;
+static int start_members(pgp_batcher* bt, Lane& L, std::vector<BItem*>& fresh) {
+  if (fresh.empty()) return 0;
   const Batch bn = make_batch(L, 0, fresh);
-  write_probs(bt, L, fresh, bn.dprobs, 0, 1, L.s);
+  const int rc = write_probs(bt, L, fresh, bn.dprobs, 0, 1, L.s);
+  if (rc) return rc;
   launch_gram_self(bn, L.s);
   launch_vec_op(bn, OP_ZERO_Z, L.s);
   for (BItem* b : fresh) b->phase = 1;
+  return check_launch();
+}
+
+// Status word of each member into slot 4 of its 8 gathered doubles.
+__global__ void k_gather_info(Prob* const* arr, int n, double* gather) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gather[8 * i + 4] = (double)arr[i]->info[0];
+}
+static void launch_gather_info(const Batch& b, double* gather, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_info, dim3((b.count + 255) / 256), dim3(256), 0, s, b.dprobs,
+                     b.count, gather);
 }
 
 // Enqueue on the lane's finalisation stream the final state (f, W, log p,
 // B factored at W) and the predict of the members in `conv`; 8 result
 // doubles per member land in the lane's gather_fin, copied to its hfin.
-static void finalize_and_predict(pgp_batcher* bt, Lane& L, std::vector<BItem*>& conv) {
+static int finalize_and_predict(pgp_batcher* bt, Lane& L, std::vector<BItem*>& conv) {
   hipStream_t s = L.s2;
   const int prec = bt->opts.precision;
   std::vector<BItem*> conv_fac, conv_fac32, pred;
@@ -1246,13 +1364,17 @@ static void finalize_and_predict(pgp_batcher* bt, Lane& L, std::vector<BItem*>& 
   }
   if (!pred.empty()) {
     const Batch bp = make_batch(L, 5, pred);
-    write_probs(bt, L, pred, bp.dprobs, 1, 0, s);
+    const int rc = write_probs(bt, L, pred, bp.dprobs, 1, 0, s);
+    if (rc) return rc;
     const int maxMp = (int)round_up(maxM, TILE);
     launch_cross_prep(bp, prec, maxM, s);
     launch_trsm(bp, prec, maxMp, 0, s);
     launch_predict_final(bp, prec, maxM, s);
+    // the status words again: the predict solve's bounded waits may have run out
+    launch_gather_info(bc, L.gather_fin, s);
   }
-  (void)hipMemcpyAsync(L.hfin, L.gather_fin, conv.size() * 64, hipMemcpyDeviceToHost, s);
+  HIP_OK(hipMemcpyAsync(L.hfin, L.gather_fin, conv.size() * 64, hipMemcpyDeviceToHost, s));
+  return check_launch();
 }
 
 // Results of finalised members from their 8 gathered doubles each.
@@ -1297,13 +1419,8 @@ static bool escalate(const pgp_batcher* bt, BItem* b) {
   return failed;
 }
 
-__global__ void k_clear_not_pd(Prob* const* arr, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) arr[i]->info[0] = 0;
-}
-
-// Clear the flags of the members of `v` escalated in their previous step
-// (array 0, after the admission's launches on the same stream).
+// Clear the non-positive-pivot bit of the members of `v` escalated in their
+// previous step (array 0, after the admission's launches on the same stream).
 static void clear_escalated(Lane& L, const std::vector<BItem*>& v) {
   std::vector<BItem*> clr;
   for (BItem* b : v)
@@ -1312,9 +1429,7 @@ static void clear_escalated(Lane& L, const std::vector<BItem*>& v) {
       b->clear = false;
     }
   if (clr.empty()) return;
-  const Batch bz = make_batch(L, 0, clr);
-  hipLaunchKernelGGL(k_clear_not_pd, dim3((bz.count + 255) / 256), dim3(256), 0, L.s, bz.dprobs,
-                     bz.count);
+  launch_clear_not_pd(make_batch(L, 0, clr), L.s);
 }
 
 static bool converged(const pgp_batcher* bt, const BItem* b) {
@@ -1345,7 +1460,16 @@ static int lane_launch(pgp_batcher* bt, int li) {
     if (b.phase == 0) fresh.push_back(&b);
     if (b.phase == 2) L.fin_list.push_back(&b);
   }
-  start_members(bt, L, fresh);
+  if (fresh.empty() && L.fin_list.empty()) {
+    bool any = false;
+    for (BItem& b : bt->items) any = any || (b.lane == li && b.phase == 1);
+    if (!any) return 0;
+  }
+  // the lane's work follows what the caller enqueued on its stream before
+  // this launch (input conversions, reuse of freed blocks)
+  HIP_OK(hipStreamWaitEvent(s, bt->ev_caller, 0));
+  int rc = start_members(bt, L, fresh);
+  if (rc) return rc;
   for (BItem& b : bt->items)
     if (b.lane == li && b.phase == 1) L.act_list.push_back(&b);
   if (L.fin_list.empty() && L.act_list.empty()) return 0;
@@ -1367,7 +1491,8 @@ static int lane_launch(pgp_batcher* bt, int li) {
     // Newton step; s joins it before the completion event
     HIP_OK(hipEventRecord(L.ev_fork, s));
     HIP_OK(hipStreamWaitEvent(L.s2, L.ev_fork, 0));
-    finalize_and_predict(bt, L, L.fin_list);
+    rc = finalize_and_predict(bt, L, L.fin_list);
+    if (rc) return rc;
     HIP_OK(hipEventRecord(L.ev_join, L.s2));
     for (BItem* b : L.fin_list) b->phase = 3;
   }
@@ -1379,7 +1504,7 @@ static int lane_launch(pgp_batcher* bt, int li) {
     newton_iter(ba, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
     HIP_OK(hipMemcpyAsync(L.hstep, L.gather, L.act_list.size() * 8, hipMemcpyDeviceToHost, s));
   }
-  int rc = check_launch();
+  rc = check_launch();
   if (rc) return rc;
   if (fin) HIP_OK(hipStreamWaitEvent(s, L.ev_join, 0));
   HIP_OK(hipEventRecord(L.ev, s));
@@ -1439,6 +1564,8 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
   b->cap = capacity;
   lane_bounds(b->lane_np);
   bool ok = true;
+  b->caller = (hipStream_t)stream;
+  ok = hipEventCreateWithFlags(&b->ev_caller, hipEventDisableTiming) == hipSuccess;
   for (int li = 0; li < NLANES; ++li) {
     Lane& L = b->lane[li];
     char* t = (char*)table + li * lane_table_bytes(capacity);
@@ -1452,12 +1579,10 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
     L.ticket2 = L.ticket + 32;
     off += 256;
     L.ddesc = (Prob*)(t + off);
-    if (li == 0) {
-      L.s = (hipStream_t)stream;
-    } else {
-      ok = ok && hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking) == hipSuccess;
-      L.own_s = ok;
-    }
+    // every lane runs on streams of its own (default priority); the caller's
+    // stream only orders the lanes' work after the caller's
+    ok = ok && hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking) == hipSuccess;
+    L.own_s = ok;
     ok = ok && hipMemsetAsync(L.ticket, 0, 256, (hipStream_t)stream) == hipSuccess &&
          hipHostMalloc((void**)&L.hstep, (size_t)capacity * 8, hipHostMallocDefault) ==
              hipSuccess &&
@@ -1526,17 +1651,20 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   if (!n_done) return fail(-4, "n_done is NULL");
   *n_done = 0;
   if (pgp_batcher_launched(bt)) return fail(-1, "pgp_batcher_step: an asynchronous launch is pending");
-  // every member in one step, on lane 0's streams
+  // every member in one step, on lane 0's streams, after the caller's work
   Lane& L = bt->lane[0];
   hipStream_t s = L.s;
   const int prec = bt->opts.precision;
   admit(bt);
+  if (bt->items.empty()) return 0;
+  HIP_OK(hipEventRecord(bt->ev_caller, bt->caller));
+  HIP_OK(hipStreamWaitEvent(s, bt->ev_caller, 0));
   std::vector<BItem*> fresh, active, fac, fac32, nofac;
   for (BItem& b : bt->items)
     if (b.phase == 0) fresh.push_back(&b);
-  start_members(bt, L, fresh);
+  int rc = start_members(bt, L, fresh);
+  if (rc) return rc;
   for (BItem& b : bt->items) active.push_back(&b);
-  if (active.empty()) return 0;
   clear_escalated(L, active);
   for (BItem* b : active) {
     if (!b->gauss || b->iter == 0 || b->refac)
@@ -1550,7 +1678,7 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   const Batch bf32 = make_batch(L, 6, fac32);
   const Batch bn = make_batch(L, 8, nofac);
   newton_iter(ba, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
-  int rc = check_launch();
+  rc = check_launch();
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(L.hstep, L.gather, active.size() * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
@@ -1559,8 +1687,7 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
     if (advance(bt, active[i], L.hstep[i])) conv.push_back(active[i]);
   if (conv.empty()) return 0;
   // s is idle (synchronised above), so s2 starts after the Newton step
-  finalize_and_predict(bt, L, conv);
-  rc = check_launch();
+  rc = finalize_and_predict(bt, L, conv);
   if (rc) return rc;
   HIP_OK(hipStreamSynchronize(L.s2));
   report(bt, conv, L.hfin, done_tags, max_done, n_done);
@@ -1571,6 +1698,8 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
 int pgp_batcher_launch(pgp_batcher_t* bt) {
   if (!bt) return fail(-1, "batcher is NULL");
   admit(bt);
+  if (bt->items.empty()) return 0;
+  HIP_OK(hipEventRecord(bt->ev_caller, bt->caller));
   for (int li = 0; li < NLANES; ++li) {
     if (bt->lane[li].launched) continue;
     const int rc = lane_launch(bt, li);
@@ -1632,6 +1761,7 @@ int pgp_batcher_destroy(pgp_batcher_t* bt) {
     if (L.hfin) (void)hipHostFree(L.hfin);
     if (L.hdesc) (void)hipHostFree(L.hdesc);
   }
+  if (bt->ev_caller) (void)hipEventDestroy(bt->ev_caller);
   delete bt;
   return 0;
 }

@@ -640,7 +640,10 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
   __shared__ __attribute__((aligned(16))) double arena[2][TILE][TILE + 1];
   __shared__ int s_val;
   const int t = take_ticket(ticket, &s_val);
-  if ((unsigned)t >= gridDim.x) return;  // counter not zero at launch
+  if ((unsigned)t >= gridDim.x) {  // counter not zero at launch: nothing to take
+    flag_members_stalled(probs, nd);  // the diagonal workgroups' members: every member
+    return;
+  }
   const int64_t c0 = (int64_t)j * TILE;
   if (t < nd) {
     const Prob& P = *probs[t];

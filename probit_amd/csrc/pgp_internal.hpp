@@ -51,10 +51,19 @@ enum Red {
   NRED = 16
 };
 
+// What a workspace's factor storage holds (Prob::state), so that entry points
+// acting on a fitted workspace can refuse one that holds something else.
+enum ResidentState {
+  RS_NONE = 0,     // nothing usable
+  RS_LAPLACE = 1,  // B = I + W^1/2 K W^1/2 at a Laplace (or caller-given) state
+  RS_VB = 2,       // B at W = 1/sigma^2 after a VB fit (objective_VB pieces valid)
+  RS_VBMAP = 3     // sigma^2 I + K (after the VB map): not a predict state
+};
+
 struct Prob {
   int32_t N, Np, D, prec;
   int32_t kern_kind, periodic, lik_kind, J;
-  int32_t sp_bounds, M, Mp, pad0;
+  int32_t sp_bounds, M, Mp, state;
   double variance, noise_std;
   double ls[PGP_MAX_DIM];           // lengthscales
   double inv_ls2[PGP_MAX_DIM];      // 1 / l^2
@@ -194,6 +203,14 @@ __device__ __forceinline__ int take_ticket(uint32_t* ticket, int* lds_slot) {
   }
   __syncthreads();
   return __builtin_amdgcn_readfirstlane(*lds_slot);
+}
+
+// A launch whose ticket counter was not zero at its start has no blocks to
+// hand out: its first n members (every member of the launch) are flagged
+// PGP_INFO_STALLED, so their results are reported invalid, not silently
+// skipped.  All threads of the workgroup call it.
+__device__ __forceinline__ void flag_members_stalled(Prob* const* probs, int n) {
+  for (int m = threadIdx.x; m < n; m += blockDim.x) atomicOr(probs[m]->info, PGP_INFO_STALLED);
 }
 
 // Wait (thread 0, bounded) until a member's progress word shows at least

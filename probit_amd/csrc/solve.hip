@@ -135,7 +135,10 @@ __global__ __launch_bounds__(NTHREADS) void k_trsv_wave(Prob* const* probs, Sche
   }
   __syncthreads();
   const int tk = __builtin_amdgcn_readfirstlane(s_val);
-  if ((unsigned)tk >= gridDim.x) return;  // counter not zero at launch: no block to take
+  if ((unsigned)tk >= gridDim.x) {  // counter not zero at launch: no block to take
+    flag_members_stalled(probs, rows.pre[1] - rows.pre[0]);  // step 0 lists every member
+    return;
+  }
   int m;
   const int local = sched_member(rows, tk, m);  // step of this member's sweep
   const Prob& P = *probs[m];
@@ -486,7 +489,10 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm_rows(Prob* const* probs, Sche
   __shared__ int s_val;
   typedef typename Acc<T>::type acc_t;
   const int tk = take_ticket(ticket, &s_val);
-  if ((unsigned)tk >= gridDim.x) return;  // counter not zero at launch: no block to take
+  if ((unsigned)tk >= gridDim.x) {  // counter not zero at launch: no block to take
+    flag_members_stalled(probs, (rows.pre[1] - rows.pre[0]) / nct);  // row 0: every member
+    return;
+  }
   int local;
   const int i = sched_member(rows, tk, local);
   const int m = local / nct, ct = local - m * nct;

@@ -157,6 +157,21 @@ CASES = [
         _kernel(L), _gauss(L), None, FAKE, 0, FAKE, 0, None), -3, "grad_out is NULL"),
     ("vb_grad: workspace NULL", lambda L, lib: lib.pgp_vb_grad(
         _kernel(L), _gauss(L), FAKE, None, 0, FAKE, 0, None), -4, "workspace is NULL"),
+    ("predict: kernel NULL", lambda L, lib: lib.pgp_predict(None, FAKE, 10, FAKE, FAKE, FAKE, 0,
+                                                            FAKE, 0, None), -1,
+     "kernel is NULL"),
+    ("covariance: kernel NULL", lambda L, lib: lib.pgp_predict_covariance(
+        None, FAKE, 10, FAKE, 10, FAKE, 0, FAKE, 0, None), -1, "kernel is NULL"),
+    ("laplace_grad: kernel NULL", lambda L, lib: lib.pgp_laplace_grad(
+        None, _gauss(L), FAKE, FAKE, 0, FAKE, 0, None), -1, "kernel is NULL"),
+    ("laplace_grad: likelihood NULL", lambda L, lib: lib.pgp_laplace_grad(
+        _kernel(L), None, FAKE, FAKE, 0, FAKE, 0, None), -2, "likelihood is NULL"),
+    ("vb_grad: kernel NULL", lambda L, lib: lib.pgp_vb_grad(
+        None, _gauss(L), FAKE, FAKE, 0, FAKE, 0, None), -1, "kernel is NULL"),
+    ("vb_grad: likelihood NULL", lambda L, lib: lib.pgp_vb_grad(
+        _kernel(L), None, FAKE, FAKE, 0, FAKE, 0, None), -2, "likelihood is NULL"),
+    ("status mapping: result NULL", lambda L, lib: lib.pgp_fit_result_apply_info(None, 1), -1,
+     "result is NULL"),
     ("gram: kernel NULL", lambda L, lib: lib.pgp_gram(None, 0, FAKE, 10, FAKE, 10, FAKE, 10, None),
      -1, "kernel is NULL"),
     ("gram: dtype", lambda L, lib: lib.pgp_gram(_kernel(L), 2, FAKE, 10, FAKE, 10, FAKE, 10, None),
@@ -254,3 +269,28 @@ def test_python_layer_raises_on_a_rejected_call():
     rc = lib.pgp_gram(None, 0, FAKE, 10, FAKE, 10, FAKE, 10, None)
     with pytest.raises(RuntimeError, match="kernel is NULL"):
         L.check(rc, "pgp_gram")
+
+
+@pytest.mark.parametrize("word,prior,bits,nan", [
+    (0, 0, 0, False),
+    (1, 0, 1, True),       # non-positive pivot: NaN objective
+    (8, 0, 8, True),       # an expired bounded wait: NaN objective
+    (9, 0, 9, True),
+    (16, 0, 0, False),     # other bits of the word are not status of the numbers
+    (0, 2, 2, True),       # negative W found on the host: NaN objective
+    (0, 4, 4, False),      # maxiter alone leaves the objective a number
+    (0, 16, 16, False),    # BF16 escalation is informational
+    (8, 16, 24, True),
+])
+def test_fit_status_mapping(word, prior, bits, nan):
+    """The device status word of a fit becomes result.status bits and a NaN
+    objective exactly where the numbers are invalid (host only, no GPU)."""
+    L, lib = _lib()
+    r = L.FitResultT()
+    r.status = prior
+    r.objective = 12.5
+    assert lib.pgp_fit_result_apply_info(C.byref(r), word) == 0
+    assert r.status == bits
+    assert math.isnan(r.objective) == nan
+    if not nan:
+        assert r.objective == 12.5

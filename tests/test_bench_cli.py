@@ -26,5 +26,19 @@ def test_help_lists_the_measurement_options():
     r = _run("--help")
     assert r.returncode == 0
     for opt in ("--steps", "--warmup", "--gpus", "--precision", "--min-n", "--plain-sample",
-                "--open-loop", "--cpu-seconds", "--copy-calibration"):
+                "--open-loop", "--cpu-strata", "--cpu-study", "--profile-window",
+                "--copy-calibration"):
         assert opt in r.stdout, opt
+
+
+def test_cpu_study_runs_without_a_gpu():
+    """The CPU-baseline plan on a tiny stream: both oracle variants, all cores
+    and one, one JSON line."""
+    import json
+    r = _run("--cpu-study", "1", "--cpu-strata", "2", "--stream", "6", "--steps", "1",
+             "--warmup", "0", "--requests-per-step", "6")
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)["cpu_study"]
+    assert set(out["runs"]) == {"lu_all_cores", "lu_1_cores", "bform_all_cores", "bform_1_cores"}
+    assert out["runs"]["lu_1_cores"]["cores"] == 1
+    assert all(v["posteriors_per_s_median"] > 0 for v in out["runs"].values())

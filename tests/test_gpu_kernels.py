@@ -98,6 +98,43 @@ def test_likelihood_eval_matches_oracle(lik):
         assert ok.all(), (lik, name, np.max(np.abs(g - r)))
 
 
+@pytest.mark.parametrize("single", [1, 0], ids=["single_bounds", "double_bounds"])
+def test_closed_form_probit_every_regime_matches_oracle(single):
+    """The BOUNDS derivatives at latent values that put each class interval's
+    z-scores in every regime of _safe_Z (exact, two-sided and one-sided tails,
+    linear; upper and lower; narrow, medium and wide classes, both ends
+    unbounded for the extreme classes), against the oracle."""
+    torch, _lib, lib = _setup()
+    cut = np.array([-np.inf, -1.0, -0.8, 0.0, 2.0, np.inf])
+    sigma = 0.5
+    grid = np.linspace(-6.0, 8.0, 2801)
+    f = np.tile(grid, 5)
+    y = np.repeat(np.arange(5, dtype=np.float64), grid.size)
+    N = f.size
+    ls = _lib.LikelihoodT()
+    ls.kind = 2
+    ls.J = 5
+    ls.noise_std = sigma
+    ls.single_precision_bounds = single
+    for c in range(6):
+        ls.cutpoints[c] = cut[c]
+    fd = torch.as_tensor(f, device="cuda")
+    yd = torch.as_tensor(y, device="cuda")
+    d1, d2 = torch.empty_like(fd), torch.empty_like(fd)
+    rc = lib.pgp_likelihood_eval(ls, fd.data_ptr(), yd.data_ptr(), N, None, d1.data_ptr(),
+                                 d2.data_ptr(), _lib.stream_handle())
+    _lib.check(rc, "pgp_likelihood_eval")
+    torch.cuda.synchronize()
+    lp = (sigma, cut)
+    r1 = O.grad_log_probit_likelihood(f, y, lp, single_precision=bool(single))
+    r2 = O.hessian_log_probit_likelihood(f, y, lp, single_precision=bool(single))
+    for name, got, r in (("d1", d1, r1), ("d2", d2, r2)):
+        g = got.cpu().numpy()
+        ok = np.isclose(g, r, rtol=1e-11, atol=1e-12) | (np.isnan(g) & np.isnan(r))
+        bad = np.flatnonzero(~ok)
+        assert ok.all(), (name, single, f[bad[:5]], y[bad[:5]], g[bad[:5]], r[bad[:5]])
+
+
 def _spd(rng, N):
     X = rng.normal(size=(N, 2))
     K = O.gram(O.KernelSpec.make(O.EQ, 1.0, 0.7, 2), X)

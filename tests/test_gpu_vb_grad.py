@@ -62,11 +62,13 @@ def test_vb_gradient_matches_finite_differences(lik, kind, period, noise):
 def test_vb_gradient_keeps_the_predict_state():
     """predict after value_and_grad equals predict right after the fit."""
     import probit_amd as P
-    X, y, theta, lp, _ = small_problem(41, 150, 2, "probit", "EQ")
+    # sigma >= 0.5: the VB fixed-point iteration contracts
+    X, y, theta, lp, _ = small_problem(41, 150, 2, "probit", "EQ", noise=0.6)
     gp = P.VBGP(data=(X, y), prior=make_prior("EQ"), log_likelihood=P.log_probit_likelihood,
-                tolerance=1e-10, maxiter=100000)
+                tolerance=1e-10, maxiter=2000)
     Xt = np.random.default_rng(42).normal(size=(70, 2))
     w, p = gp.approximate_posterior((theta, lp))
+    assert gp.last_result.status == 0, gp.last_result.status  # converged
     m0, v0 = gp.predict(Xt, (theta, lp), w, p)
     m0, v0 = m0.cpu().numpy().copy(), v0.cpu().numpy().copy()
     gp.value_and_grad()((theta, lp))
