@@ -259,7 +259,7 @@ def _mix(problems):
 
 TIMER_CLASSES = (("outer_update", "TIMER_SYRK", "mfma"), ("column_steps", "TIMER_PANEL", "mfma"),
                  ("trsv", "TIMER_TRSV", "hbm"), ("gram", "TIMER_GRAM", "hbm"),
-                 ("trsm", "TIMER_TRSM", "mfma"))
+                 ("trsm", "TIMER_TRSM", "mfma"), ("variance_refine", "TIMER_REFINE", "mfma"))
 
 
 def read_timer(lib, _lib, cls):

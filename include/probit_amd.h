@@ -137,7 +137,8 @@ int pgp_fit_result_apply_info(pgp_fit_result_t* result, int32_t info_word);
 /* Opt-in launch timer for measurement: with bit c of `mask` set, launches of
  * kernel class c (0 = outer trailing SYRK update of the Cholesky, 1 = its
  * per-column steps: diagonal factor, panel, column update, 2 = TRSV,
- * 3 = Gram assembly, 4 = TRSM) are bracketed by HIP events on their stream,
+ * 3 = Gram assembly, 4 = TRSM, 5 = the FP64 correction of the predictive
+ * variance) are bracketed by HIP events on their stream,
  * with the algorithmic flops / bytes each launch performs.  pgp_timer_read
  * waits for the recorded events and returns the class totals.  Not
  * thread-safe; no counterpart in the reference (measurement only). */
@@ -145,6 +146,32 @@ int pgp_timer_enable(uint32_t mask);
 int pgp_timer_reset(void);
 int pgp_timer_read(int32_t cls, int64_t* launches, double* total_ms,
                    double* algo_flops, double* algo_bytes);
+
+/* Schedule table of the blocked Cholesky (the autotuner's table; no
+ * counterpart in the reference, whose factorisation is LAPACK's inside XLA:
+ * probit/implicit/Laplace.py:24).  One entry per precision (PGP_PREC_*) and
+ * size class: class 0 holds problems with padded N <= PGP_TUNE_CLASS_NP,
+ * class 1 the larger ones, and every factorisation of a batch runs each
+ * class with its own entry, so a problem's schedule depends only on its own
+ * size.  superstep (1..16): block columns per super-step (rank-64 superstep
+ * trailing updates; changes the rounding of FP32 / BF16 factors);
+ * column_max: super-steps whose first column has at most this many
+ * workgroups run one launch per block column (0: never); fuse_max: update
+ * launches of at most this many tiles factor the next diagonal block inside
+ * the launch (0: never).  column_max and fuse_max change only the launches,
+ * never the results.  The table is process-wide, read at every
+ * factorisation and guarded by a mutex; probit_amd.autotune fills it from
+ * timed trials and persists it. */
+#define PGP_TUNE_CLASSES 2
+#define PGP_TUNE_CLASS_NP 1024
+typedef struct pgp_tune_entry {
+  int32_t superstep;
+  int32_t column_max;
+  int32_t fuse_max;
+  int32_t reserved;
+} pgp_tune_entry_t;
+int pgp_tune_get(int32_t precision, int32_t size_class, pgp_tune_entry_t* out);
+int pgp_tune_set(int32_t precision, int32_t size_class, const pgp_tune_entry_t* entry);
 
 /* ------------------------------------------------------------------ */
 /* kernel-level entry points                                            */
@@ -172,6 +199,21 @@ int pgp_likelihood_eval(const pgp_likelihood_t* lik, const double* f,
  * (probit/utilities.py:232-249). */
 int pgp_probit_predictive(const pgp_likelihood_t* lik, const double* mean,
                           const double* var, int64_t M, double* out, void* stream);
+
+/* Ordinal test metrics of the reference's classification example
+ * (calculate_metrics, examples/classification.py:325-341) from class
+ * probabilities P (M x J, row-major: pgp_probit_predictive's output) and
+ * test labels y_test (M class indices as doubles): y_pred (M int32) =
+ * argmax_j P_ij (first maximum); predictive_likelihood (M x M row-major, may
+ * be NULL) = P[:, y_test] -- the reference's column indexing, an M x M
+ * matrix, not the per-point probabilities; metrics (5 device doubles) =
+ * [incorrect count, correct count, mean absolute error, log predictive
+ * probability = sum over all M x M entries of log predictive_likelihood,
+ * mean zero-one error].  workspace: 3 M caller-owned doubles.  A label
+ * outside 0..J-1 gives NaN entries. */
+int pgp_ordinal_metrics(const double* P, int32_t J, const double* y_test, int64_t M,
+                        int32_t* y_pred, double* predictive_likelihood, double* metrics,
+                        double* workspace, void* stream);
 
 /* In-place lower Cholesky A = L L^T of an N x N matrix (N % 64 == 0),
  * storage double (PGP_PREC_FP64) or float (FP32/BF16 modes).  dinv
@@ -250,6 +292,10 @@ int64_t pgp_predict_workspace_size(int64_t N, int64_t M, int32_t precision);
 
 /* Approximator.predict (probit/approximators.py:154-180) from the resident
  * factor: mean = Kfs^T a, var = Kss - ||L_B^-1 W^1/2 Kfs||^2 (latent).
+ * With a factor stored in FP32 (modes fp32, bf16) the variance is corrected
+ * in FP64: var = Kss - (2 s^T z - z^T B z), z = (L L^T)^-1 s from the FP32
+ * solves and B z from the FP64 Gram matrix, whose error is quadratic in
+ * that of z (PGP_PREDICT_REFINE=0 in the environment turns it off).
  * `kernel` must be the resident fit's kernel (the reference's predict takes
  * the parameters it uses): other parameters, or a workspace that holds no
  * predict state (nothing fitted, the VB map's sigma^2 I + K), give a

@@ -337,7 +337,8 @@ def hessian_log_probit_likelihood(f, y, lik, single_precision=True):
 # ----------------------------------------------------------------------------
 GAUSSIAN = "gaussian"
 PROBIT = "probit"
-PROBIT_CLOSED = "probit_closed"
+PROBIT_CLOSED = "probit_closed"  # BOUNDS["single"] (the callables' default)
+PROBIT_CLOSED_DOUBLE = "probit_closed_double"  # single_precision=False: BOUNDS["double"]
 
 
 def lik_derivs(kind, f, y, lik, order=2):
@@ -345,13 +346,14 @@ def lik_derivs(kind, f, y, lik, order=2):
         return gaussian_derivs(f, y, lik, order)
     if kind == PROBIT:
         return probit_derivs(f, y, lik, order)
-    if kind == PROBIT_CLOSED:
+    if kind in (PROBIT_CLOSED, PROBIT_CLOSED_DOUBLE):
         if order > 2:
             raise NotImplementedError("closed-form probit: value path only")
+        sp = kind == PROBIT_CLOSED
         return {
             "ll": log_probit_likelihood(f, y, lik),
-            "d1": grad_log_probit_likelihood(f, y, lik),
-            "d2": hessian_log_probit_likelihood(f, y, lik),
+            "d1": grad_log_probit_likelihood(f, y, lik, single_precision=sp),
+            "d2": hessian_log_probit_likelihood(f, y, lik, single_precision=sp),
         }
     raise ValueError(kind)
 
@@ -574,6 +576,24 @@ def probit_predictive_distributions(lik, mean, variance):
         hi = 1.0 if cut[j + 1] == np.inf else ndtr((cut[j + 1] - mean) / std)
         out[:, j] = hi - lo
     return out
+
+
+def calculate_metrics(y_test, predictive_distributions):
+    """examples/classification.py:325-341: argmax predictions and the reference's
+    metrics, with its column indexing predictive_distributions[:, y_test]
+    (an M x M matrix whose log is summed).  Returns (mean_zero_one,
+    mean_absolute_error, log_predictive_probability, predictive_likelihood,
+    y_pred)."""
+    P = np.asarray(predictive_distributions, dtype=np.float64)
+    y = np.asarray(y_test).astype(np.int64)
+    y_pred = np.argmax(P, axis=1)
+    predictive_likelihood = P[:, y]
+    mean_absolute_error = np.sum(np.abs(y_pred - y)) / len(y)
+    with np.errstate(divide="ignore"):
+        log_predictive_probability = np.sum(np.log(predictive_likelihood))
+    mean_zero_one = np.sum(y_pred != y) / len(y)
+    return mean_zero_one, mean_absolute_error, log_predictive_probability, \
+        predictive_likelihood, y_pred
 
 
 # ----------------------------------------------------------------------------

@@ -10,7 +10,8 @@ HIP kernels behind a C ABI, ``include/probit_amd.h``); build it with
 from .approximators import Approximator, LaplaceGP, VBGP
 from .kernels import EQ, Kernel, Matern12
 from .optimize import minimise_l_bfgs_b
-from .utilities import (BOUNDS, CutpointValueError, InvalidKernel, check_cutpoints, check_data,
+from .utilities import (BOUNDS, CutpointValueError, InvalidKernel, calculate_metrics,
+                        check_cutpoints, check_data,
                         grad_log_probit_likelihood, h, hessian_log_probit_likelihood,
                         log_gaussian_likelihood, log_probit_likelihood, ndtr, norm_cdf,
                         norm_logpdf, norm_pdf, norm_z_logpdf, norm_z_pdf, probit,
@@ -22,5 +23,5 @@ __all__ = [
     "grad_log_probit_likelihood", "hessian_log_probit_likelihood", "h",
     "log_gaussian_likelihood", "log_probit_likelihood", "probit_predictive_distributions",
     "ndtr", "norm_cdf", "norm_pdf", "norm_z_pdf", "norm_logpdf", "norm_z_logpdf", "probit",
-    "probit_likelihood", "minimise_l_bfgs_b",
+    "probit_likelihood", "minimise_l_bfgs_b", "calculate_metrics",
 ]

@@ -38,6 +38,7 @@ TIMER_PANEL = 1
 TIMER_TRSV = 2
 TIMER_GRAM = 3
 TIMER_TRSM = 4
+TIMER_REFINE = 5
 
 LIB_PATH = Path(__file__).resolve().parent / "libprobit_amd.so"
 
@@ -117,6 +118,19 @@ class BatchItemT(C.Structure):
     ]
 
 
+TUNE_CLASSES = 2
+TUNE_CLASS_NP = 1024
+
+
+class TuneEntryT(C.Structure):
+    _fields_ = [
+        ("superstep", C.c_int32),
+        ("column_max", C.c_int32),
+        ("fuse_max", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -124,6 +138,8 @@ _I32 = C.c_int32
 _SIGNATURES = {
     "pgp_last_error": (C.c_char_p, []),
     "pgp_version": (C.c_int, []),
+    "pgp_tune_get": (C.c_int, [_I32, _I32, C.POINTER(TuneEntryT)]),
+    "pgp_tune_set": (C.c_int, [_I32, _I32, C.POINTER(TuneEntryT)]),
     "pgp_fit_result_apply_info": (C.c_int, [C.POINTER(FitResultT), C.c_int32]),
     "pgp_timer_enable": (C.c_int, [C.c_uint32]),
     "pgp_timer_reset": (C.c_int, []),
@@ -132,6 +148,7 @@ _SIGNATURES = {
     "pgp_gram": (C.c_int, [C.POINTER(KernelT), C.c_int, _P, _I64, _P, _I64, _P, _I64, _P]),
     "pgp_likelihood_eval": (C.c_int, [C.POINTER(LikelihoodT), _P, _P, _I64, _P, _P, _P, _P]),
     "pgp_probit_predictive": (C.c_int, [C.POINTER(LikelihoodT), _P, _P, _I64, _P, _P]),
+    "pgp_ordinal_metrics": (C.c_int, [_P, _I32, _P, _I64, _P, _P, _P, _P, _P]),
     "pgp_potrf": (C.c_int, [C.c_int, _P, _I64, _I64, _P, _P, _P, _P]),
     "pgp_prior_sample_workspace_size": (_I64, [_I64]),
     "pgp_prior_sample": (C.c_int, [C.POINTER(KernelT), _P, _I64, C.c_double, _P, _I64, _I64, _P,
@@ -196,11 +213,23 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
     return lib
 
 
+_TABLE_LOADED = False
+
+
 def lib() -> C.CDLL:
-    """The library, for a compute call: requires a visible GPU."""
+    """The library, for a compute call: requires a visible GPU.  The first
+    call applies the persisted schedule table for this GPU's architecture
+    (probit_amd.autotune.load: PGP_TUNE_FILE, else the package's table), so
+    every path of the process factors with the same schedule."""
+    global _TABLE_LOADED
     if not torch.cuda.is_available():
         raise RuntimeError("probit_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
-    return load()
+    L = load()
+    if not _TABLE_LOADED:
+        _TABLE_LOADED = True
+        from . import autotune
+        autotune.load()
+    return L
 
 
 def check(rc: int, what: str) -> None:

@@ -19,6 +19,7 @@ with ``strict=True`` they raise ``FloatingPointError`` instead.
 from __future__ import annotations
 
 import ctypes as C
+import functools
 import warnings
 from abc import ABC, abstractmethod
 
@@ -34,18 +35,35 @@ from .utilities import (InvalidKernel, grad_log_probit_likelihood,
 __all__ = ["Approximator", "LaplaceGP", "VBGP"]
 
 
-def _resolve_likelihood(log_likelihood, grad_log_likelihood, hessian_log_likelihood) -> int:
-    """Map the reference's likelihood callables to a fused-kernel ID."""
+def _closed_form(fn, base):
+    """(True, single_precision) if `fn` is `base` or functools.partial(base,
+    single_precision=...), else (False, None)."""
+    if fn is base:
+        return True, True
+    if (isinstance(fn, functools.partial) and fn.func is base and not fn.args
+            and set(fn.keywords) <= {"single_precision"}):
+        return True, bool(fn.keywords.get("single_precision", True))
+    return False, None
+
+
+def _resolve_likelihood(log_likelihood, grad_log_likelihood, hessian_log_likelihood):
+    """Map the reference's likelihood callables to a fused-kernel ID and the
+    BOUNDS set of the closed-form derivatives (True: BOUNDS["single"], the
+    callables' default; functools.partial(..., single_precision=False) picks
+    BOUNDS["double"], as it would in the reference)."""
     if log_likelihood is log_gaussian_likelihood:
         if grad_log_likelihood is not None or hessian_log_likelihood is not None:
             raise NotImplementedError("custom derivatives of the Gaussian likelihood")
-        return _lib.LIK_GAUSSIAN
+        return _lib.LIK_GAUSSIAN, True
     if log_likelihood is log_probit_likelihood:
         if grad_log_likelihood is None and hessian_log_likelihood is None:
-            return _lib.LIK_PROBIT
-        if (grad_log_likelihood is grad_log_probit_likelihood
-                and hessian_log_likelihood is hessian_log_probit_likelihood):
-            return _lib.LIK_PROBIT_CLOSED
+            return _lib.LIK_PROBIT, True
+        g_ok, g_sp = _closed_form(grad_log_likelihood, grad_log_probit_likelihood)
+        h_ok, h_sp = _closed_form(hessian_log_likelihood, hessian_log_probit_likelihood)
+        if g_ok and h_ok:
+            if g_sp != h_sp:
+                raise NotImplementedError("the gradient and Hessian use different BOUNDS sets")
+            return _lib.LIK_PROBIT_CLOSED, g_sp
     raise NotImplementedError(
         "no fused HIP kernel for this likelihood: use probit_amd.utilities."
         "log_gaussian_likelihood or log_probit_likelihood (optionally with "
@@ -92,6 +110,18 @@ def numerical_failure(status: int, objective: float = 0.0) -> str | None:
     return ", ".join(names) or None
 
 
+def _refuse_closed_form_gradient(lik_kind):
+    """The evidence gradient is not provided for the closed-form (BOUNDS)
+    probit derivatives: JAX would differentiate through their piecewise tail
+    series (probit/utilities.py:88-192), including the branch selection, and
+    no adjoint of those pieces is built here.  The value, fit and predict are."""
+    if lik_kind == _lib.LIK_PROBIT_CLOSED:
+        raise NotImplementedError(
+            "the evidence gradient is not provided with grad_/hessian_log_probit_likelihood "
+            "(the closed-form BOUNDS derivatives); use log_probit_likelihood alone, whose "
+            "derivatives are exact")
+
+
 class Approximator(ABC):
     """Base class: data upload, parameter packing, workspaces, predict."""
 
@@ -105,8 +135,8 @@ class Approximator(ABC):
         self.tolerance = float(tolerance)
         self.strict = bool(strict)
         self.prior = prior
-        self.lik_kind = _resolve_likelihood(log_likelihood, grad_log_likelihood,
-                                            hessian_log_likelihood)
+        self.lik_kind, self._single_bounds = _resolve_likelihood(
+            log_likelihood, grad_log_likelihood, hessian_log_likelihood)
         self.log_likelihood = log_likelihood
         self.grad_log_likelihood = grad_log_likelihood
         self.hessian_log_likelihood = hessian_log_likelihood
@@ -148,7 +178,7 @@ class Approximator(ABC):
     def _structs(self, parameters):
         kern = self._kernel(parameters[0])
         ks = kern.to_struct(self.D)
-        ls = likelihood_struct(self.lik_kind, parameters[1])
+        ls = likelihood_struct(self.lik_kind, parameters[1], self._single_bounds)
         return ks, ls
 
     @staticmethod
@@ -519,6 +549,7 @@ class LaplaceGP(Approximator):
         return w, p
 
     def _grad(self, ks, ls):
+        _refuse_closed_form_gradient(self.lik_kind)
         lib = _lib.lib()
         ws = self._workspace()
         gws = self._grad_workspace()
@@ -626,6 +657,7 @@ class VBGP(Approximator):
         return f
 
     def _grad(self, ks, ls):
+        _refuse_closed_form_gradient(self.lik_kind)
         lib = _lib.lib()
         ws = self._workspace()
         gws = self._grad_workspace()

@@ -103,7 +103,7 @@ static WsLayout ws_layout(int64_t N, int prec) {
 }
 
 struct PredLayout {
-  int64_t Mp, S, mupart, total;
+  int64_t Mp, S, mupart, qpart, total;
 };
 static PredLayout pred_layout(int64_t Np, int64_t M, int prec) {
   PredLayout p;
@@ -113,8 +113,35 @@ static PredLayout pred_layout(int64_t Np, int64_t M, int prec) {
   off += align256(Np * p.Mp * elem_size(prec));
   p.mupart = off;
   off += align256((Np / TILE) * p.Mp * 8);
+  p.qpart = off;
+  off += align256((Np / TILE) * p.Mp * 8);
   p.total = off;
   return p;
+}
+
+// Factors stored in FP32 (modes fp32, bf16) get the FP64 correction of the
+// predictive variance (k_pred_refine) unless PGP_PREDICT_REFINE=0.
+static bool predict_refine(int prec) {
+  static const bool on = [] {
+    const char* e = getenv("PGP_PREDICT_REFINE");
+    return !(e && e[0] == '0');
+  }();
+  return on && prec != PGP_PREC_FP64;
+}
+
+// The predict of the members of `b` (descriptors with S, mupart, qpart and
+// the outputs set): S = W^1/2 Kfs with the partial means, S <- L^-1 S, and
+// either var = Kss - ||L^-1 s||^2 or, with the correction, S <- L^-T S and
+// the FP64 quadratic forms.
+static void predict_launches(const Batch& b, int prec, int maxM, hipStream_t s) {
+  const int maxMp = (int)round_up(maxM, TILE);
+  launch_cross_prep(b, prec, maxM, s);
+  launch_trsm(b, prec, maxMp, 0, s);
+  if (predict_refine(prec)) {
+    launch_trsm_t(b, prec, maxMp, s);
+    launch_pred_refine(b, prec, maxM, s);
+  }
+  launch_predict_final(b, prec, maxM, s);
 }
 
 struct GradLayout {
@@ -407,6 +434,13 @@ static void newton_iter(const Batch& all, const Batch* fac, const Batch* fac32,
 }
 
 // Final state at the returned weight: f, W, sum log p, and B factored at W.
+// In the BF16 mode the Newton iterations factor with BF16-operand updates
+// (the FP64 residual keeps their fixed point the FP64 one), but the final
+// factor -- the one predict, the log-determinant and the gradient use -- is
+// formed with FP32 updates for every member (final_fp32): a BF16-update
+// factor's error (~1e-3 relative) would pass into the predictive variance
+// and the objective.  The single-problem fit and the batcher apply the same
+// rule, so their results stay bitwise equal.
 static void laplace_finalize(const Batch& all, const Batch* fac, const Batch* fac32, int prec,
                              double* gather, hipStream_t s) {
   launch_gemv_sym(all, V_Z, V_F, 0, s);
@@ -543,6 +577,30 @@ int pgp_timer_read(int32_t cls, int64_t* launches, double* total_ms, double* alg
 }
 int pgp_version(void) { return 2; }
 
+int pgp_tune_get(int32_t precision, int32_t size_class, pgp_tune_entry_t* out) {
+  if (precision < 0 || precision > 2) return fail(-1, "unknown precision %d", precision);
+  if (size_class < 0 || size_class >= PGP_TUNE_CLASSES)
+    return fail(-2, "size class must be in [0, %d)", PGP_TUNE_CLASSES);
+  if (!out) return fail(-3, "out is NULL");
+  *out = tune_get(precision, size_class);
+  return 0;
+}
+
+int pgp_tune_set(int32_t precision, int32_t size_class, const pgp_tune_entry_t* entry) {
+  if (precision < 0 || precision > 2) return fail(-1, "unknown precision %d", precision);
+  if (size_class < 0 || size_class >= PGP_TUNE_CLASSES)
+    return fail(-2, "size class must be in [0, %d)", PGP_TUNE_CLASSES);
+  if (!entry) return fail(-3, "entry is NULL");
+  if (entry->superstep < 1 || entry->superstep > 16)
+    return fail(-3, "superstep must be in [1, 16] (got %d)", entry->superstep);
+  if (entry->column_max < 0 || entry->fuse_max < 0)
+    return fail(-3, "column_max / fuse_max must be >= 0");
+  pgp_tune_entry_t v = *entry;
+  v.reserved = 0;
+  tune_set(precision, size_class, v);
+  return 0;
+}
+
 int pgp_fit_result_apply_info(pgp_fit_result_t* result, int32_t info_word) {
   if (!result) return fail(-1, "result is NULL");
   apply_info(result, info_word);
@@ -579,6 +637,20 @@ int pgp_probit_predictive(const pgp_likelihood_t* lik, const double* mean, const
   if (!mean || !var || !out) return fail(-2, "NULL pointer");
   if (M < 0) return fail(-4, "M < 0");
   launch_probit_predictive(lik, mean, var, M, out, (hipStream_t)stream);
+  return check_launch();
+}
+
+int pgp_ordinal_metrics(const double* P, int32_t J, const double* y_test, int64_t M,
+                        int32_t* y_pred, double* predictive_likelihood, double* metrics,
+                        double* workspace, void* stream) {
+  if (!P) return fail(-1, "P is NULL");
+  if (J < 2 || J + 1 > PGP_MAX_CUT) return fail(-2, "J must be in [2, %d]", PGP_MAX_CUT - 1);
+  if (!y_test) return fail(-3, "y_test is NULL");
+  if (M <= 0 || M > (int64_t)INT32_MAX) return fail(-4, "M out of range");
+  if (!y_pred || !metrics) return fail(-5, "y_pred / metrics are NULL");
+  if (!workspace) return fail(-8, "workspace (3 M doubles) is NULL");
+  launch_ordinal_metrics(P, J, y_test, M, y_pred, predictive_likelihood, metrics, workspace,
+                         (hipStream_t)stream);
   return check_launch();
 }
 
@@ -659,24 +731,29 @@ int64_t pgp_grad_workspace_size(int64_t N, int32_t D) {
   return grad_layout(round_up(N, TILE), PGP_PREC_FP64).total;
 }
 
-int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
-                    const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
-                    pgp_fit_result_t* result, void* workspace, int64_t workspace_bytes,
-                    void* stream) {
+}  // extern "C"
+
+// Argument checks of the Laplace fit entry points (the workspace aside).
+static int check_fit_args(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+                          const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
+                          const pgp_fit_result_t* result) {
   int rc = validate(problem, kernel, lik);
   if (rc) return rc;
   if (!lik) return fail(-3, "likelihood is NULL");
   if (!opts) return fail(-4, "options are NULL");
   if (!result) return fail(-5, "result is NULL");
+  if (opts->precision < 0 || opts->precision > 2)
+    return fail(-4, "unknown precision %d", opts->precision);
+  return 0;
+}
+
+// The Laplace fit (Newton loop, final state, resident factor) in workspace
+// `ws`, whose size the caller has checked against ws_layout(N, precision).
+static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+                          const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
+                          pgp_fit_result_t* result, char* ws, hipStream_t s) {
+  int rc = 0;
   const int prec = opts->precision;
-  if (prec < 0 || prec > 2) return fail(-4, "unknown precision %d", prec);
-  if (!workspace) return fail(-6, "workspace is NULL");
-  const WsLayout L = ws_layout(problem->N, prec);
-  if (workspace_bytes < L.total)
-    return fail(-7, "workspace too small: %lld < %lld", (long long)workspace_bytes,
-                (long long)L.total);
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = (char*)workspace;
   const int refine = opts->refine >= 0 ? opts->refine : default_refine(prec);
   Prob p = make_prob(problem, kernel, lik, prec, ws);
   p.state = RS_LAPLACE;
@@ -728,7 +805,10 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   result->iterations = it;
   result->step_norm = step;
   if (step > opts->tolerance) result->status |= PGP_INFO_MAXITER;
-  laplace_finalize(b, gauss ? nullptr : &b, nullptr, cur, gather, s);
+  if (prec == PGP_PREC_BF16)  // the final factor with FP32 updates (see final_fp32)
+    laplace_finalize(b, nullptr, &b, PGP_PREC_FP32, gather, s);
+  else
+    laplace_finalize(b, gauss ? nullptr : &b, nullptr, cur, gather, s);
   rc = check_launch();
   if (rc) return rc;
   double g4[4];
@@ -738,6 +818,23 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
   HIP_OK(hipStreamSynchronize(s));
   fill_result_final(result, g4, info[0]);
   return 0;
+}
+
+extern "C" {
+
+int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+                    const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
+                    pgp_fit_result_t* result, void* workspace, int64_t workspace_bytes,
+                    void* stream) {
+  int rc = check_fit_args(problem, kernel, lik, opts, result);
+  if (rc) return rc;
+  if (!workspace) return fail(-6, "workspace is NULL");
+  const WsLayout L = ws_layout(problem->N, opts->precision);
+  if (workspace_bytes < L.total)
+    return fail(-7, "workspace too small: %lld < %lld", (long long)workspace_bytes,
+                (long long)L.total);
+  return laplace_fit_ws(problem, kernel, lik, opts, result, (char*)workspace,
+                        (hipStream_t)stream);
 }
 
 int pgp_vb_map(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
@@ -841,13 +938,12 @@ int pgp_predict(const pgp_kernel_t* kernel, const double* Xtest, int64_t M, doub
   p.Mp = (int32_t)PL.Mp;
   p.S = pw + PL.S;
   p.mupart = (double*)(pw + PL.mupart);
+  p.qpart = predict_refine(p.prec) ? (double*)(pw + PL.qpart) : nullptr;
   p.pmean = mean;
   p.pvar = var;
   write_prob(ws, p, 0, s);
   const Batch b = single_batch(ws, p.Np);
-  launch_cross_prep(b, p.prec, (int)M, s);
-  launch_trsm(b, p.prec, (int)PL.Mp, 0, s);
-  launch_predict_final(b, p.prec, (int)M, s);
+  predict_launches(b, p.prec, (int)M, s);
   rc = check_launch();
   if (rc) return rc;
   int bits = 0;
@@ -883,6 +979,7 @@ int pgp_predict_covariance(const pgp_kernel_t* kernel, const double* Xtest, int6
   p.Mp = (int32_t)PL.Mp;
   p.S = pw + PL.S;
   p.mupart = (double*)(pw + PL.mupart);
+  p.qpart = nullptr;
   p.pmean = nullptr;
   p.pvar = nullptr;
   write_prob(ws, p, 0, s);
@@ -1341,7 +1438,10 @@ static int finalize_and_predict(pgp_batcher* bt, Lane& L, std::vector<BItem*>& c
   const int prec = bt->opts.precision;
   std::vector<BItem*> conv_fac, conv_fac32, pred;
   for (BItem* b : conv)
-    if (!b->gauss) (b->esc ? conv_fac32 : conv_fac).push_back(b);
+    if (prec == PGP_PREC_BF16)
+      conv_fac32.push_back(b);  // the final factor with FP32 updates (see final_fp32)
+    else if (!b->gauss)
+      (b->esc ? conv_fac32 : conv_fac).push_back(b);
   const Batch bc = make_batch(L, 3, conv);
   const Batch bcf = make_batch(L, 4, conv_fac);
   const Batch bcf32 = make_batch(L, 7, conv_fac32);
@@ -1357,6 +1457,7 @@ static int finalize_and_predict(pgp_batcher* bt, Lane& L, std::vector<BItem*>& c
     b->p.Mp = (int32_t)PL.Mp;
     b->p.S = pw + PL.S;
     b->p.mupart = (double*)(pw + PL.mupart);
+    b->p.qpart = predict_refine(prec) ? (double*)(pw + PL.qpart) : nullptr;
     b->p.pmean = it->mean;
     b->p.pvar = it->var;
     pred.push_back(b);
@@ -1366,10 +1467,7 @@ static int finalize_and_predict(pgp_batcher* bt, Lane& L, std::vector<BItem*>& c
     const Batch bp = make_batch(L, 5, pred);
     const int rc = write_probs(bt, L, pred, bp.dprobs, 1, 0, s);
     if (rc) return rc;
-    const int maxMp = (int)round_up(maxM, TILE);
-    launch_cross_prep(bp, prec, maxM, s);
-    launch_trsm(bp, prec, maxMp, 0, s);
-    launch_predict_final(bp, prec, maxM, s);
+    predict_launches(bp, prec, maxM, s);
     // the status words again: the predict solve's bounded waits may have run out
     launch_gather_info(bc, L.gather_fin, s);
   }

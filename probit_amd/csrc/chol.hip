@@ -22,6 +22,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "mfma.hpp"
@@ -776,41 +777,56 @@ __global__ __launch_bounds__(NTHREADS) void k_dinv_raw(const T* A, int64_t ld, d
   invert_diag_block<T>(A, ld, (int64_t)blockIdx.x * TILE, dinv + (size_t)blockIdx.x * TILE * TILE);
 }
 
-// Update launches of at most this many tiles factor the next diagonal block
-// inside the launch (PGP_FUSE_MAX; 0 disables).
-static int fuse_max() {
-  static const int v = [] {
-    const char* e = getenv("PGP_FUSE_MAX");
-    return e ? atoi(e) : 512;
-  }();
-  return v;
+// ---------------------------------------------------------------------------
+// Schedule table (the autotuner's): per precision and size class, the
+// super-step width G (rank-64G outer updates), COLUMN_MAX (super-steps whose
+// first column has at most this many workgroups run one k_column launch per
+// column; 0 disables) and FUSE_MAX (update launches of at most this many
+// tiles factor the next diagonal block inside the launch; 0 disables).
+// Size class 0 holds the members with padded N <= TUNE_CLASS_NP, class 1 the
+// larger ones; a batch is factored class by class, so a member's schedule --
+// and with G its rounding -- depends only on its own size (a result is the
+// same alone and in any mix, in the batcher or the single-problem path).
+// COLUMN_MAX and FUSE_MAX change only the launches, never the arithmetic.
+// Initial values: PGP_SUPERSTEP / PGP_COLUMN_MAX / PGP_FUSE_MAX from the
+// environment, else 8 / 512 / 512.  Read at every factorisation, written
+// rarely (pgp_tune_set): a mutex guards it.
+// ---------------------------------------------------------------------------
+constexpr int TUNE_CLASSES = 2;
+constexpr int TUNE_CLASS_NP = 1024;
+
+static int env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  return e ? atoi(e) : dflt;
 }
 
-// Super-step width in 64-column blocks (PGP_SUPERSTEP, default 8): the
-// trailing matrix right of a super-step is updated once per super-step with
-// rank 64 * width, so it is read and written width times less often than
-// with rank-64 updates, and each output tile of the update amortises its
-// start-up over width chunks of products.
-static int superstep() {
-  static const int v = [] {
-    const char* e = getenv("PGP_SUPERSTEP");
-    const int w = e ? atoi(e) : 8;
-    return w < 1 ? 1 : (w > 16 ? 16 : w);
-  }();
-  return v;
+struct TuneTable {
+  std::mutex mu;
+  pgp_tune_entry_t e[3][TUNE_CLASSES];
+  TuneTable() {
+    const int G = std::min(16, std::max(1, env_int("PGP_SUPERSTEP", 8)));
+    const int cm = env_int("PGP_COLUMN_MAX", 512), fm = env_int("PGP_FUSE_MAX", 512);
+    for (int p = 0; p < 3; ++p)
+      for (int c = 0; c < TUNE_CLASSES; ++c) e[p][c] = pgp_tune_entry_t{G, cm, fm, 0};
+  }
+};
+static TuneTable& tune_table() {
+  static TuneTable t;
+  return t;
 }
 
-// One launch per block column (k_column) for the super-steps whose first
-// column has at most this many workgroups (PGP_COLUMN_MAX; 0 disables):
-// there the launch count and the per-column chain dominate; larger
-// super-steps use the separate panel / update / factor kernels, whose lean
-// register footprint keeps occupancy high.
-static int column_max() {
-  static const int v = [] {
-    const char* e = getenv("PGP_COLUMN_MAX");
-    return e ? atoi(e) : 512;
-  }();
-  return v;
+int tune_class_of(int np) { return np > TUNE_CLASS_NP ? 1 : 0; }
+
+pgp_tune_entry_t tune_get(int prec, int cls) {
+  TuneTable& t = tune_table();
+  std::lock_guard<std::mutex> lock(t.mu);
+  return t.e[prec][cls];
+}
+
+void tune_set(int prec, int cls, const pgp_tune_entry_t& v) {
+  TuneTable& t = tune_table();
+  std::lock_guard<std::mutex> lock(t.mu);
+  t.e[prec][cls] = v;
 }
 
 static bool sorted_desc(const Batch& b) {
@@ -859,9 +875,10 @@ static void column_launch(const Batch& b, int j, int s0, bool tpa, hipStream_t s
 // Both forms apply the same operations in the same order to every tile, so a
 // member's factor does not depend on which form its batch used.
 template <typename T, int BF16>
-static void potrf_steps(const Batch& b, hipStream_t s) {
+static void potrf_steps(const Batch& b, hipStream_t s, const pgp_tune_entry_t& te) {
   const int nb = b.maxNp / TILE;
-  const int G = superstep();
+  const int G = te.superstep;
+  const int column_max = te.column_max, fuse_max = te.fuse_max;
   const bool tsy = timer_on(TC_SYRK), tpa = timer_on(TC_PANEL);
   constexpr double F3 = TILE * TILE * TILE / 3.0;  // factor of one 64x64 block
   constexpr double G64 = 2.0 * TILE * TILE * TILE;  // one 64x64x64 product
@@ -872,7 +889,7 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
     for (int m = off; m < off + sc.count; ++m) f += per_member(b.np(m) / TILE);
     return f;
   };
-  const bool colok = column_max() > 0 && b.ticket && sorted_desc(b);
+  const bool colok = column_max > 0 && b.ticket && sorted_desc(b);
   // workgroups of the column launch at the first column of a super-step;
   // non-increasing in s0, so once a super-step uses the column kernel all
   // later ones do
@@ -880,7 +897,7 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
     if (!colok) return false;
     long long w = 0;
     for (int m = 0; m < b.count; ++m) w += std::max(0, b.np(m) / TILE - s0);
-    return w <= column_max();
+    return w <= column_max;
   };
   auto outer_launch = [&](int s0, int s_end, bool fuse) {
     auto outer_tiles = [&](int m) {
@@ -935,7 +952,7 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
     // super-step was factored at the end of the previous one (or by F_0).
     if (k > s0) {
       auto tiles = [&](int m) { return std::max(0, b.np(m) / TILE - k); };
-      const bool fused = total_units(tiles) <= fuse_max();
+      const bool fused = total_units(tiles) <= fuse_max;
       sched_launch(b, tiles, [&](Prob* const* dp, const Sched& sc, int n) {
         if (tpa) timer_begin(TC_PANEL, s);
         if (fused)
@@ -979,7 +996,7 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
       const bool fused = !next_col && total_units([&](int m) {
                                         const int t = std::max(0, b.np(m) / TILE - s_end);
                                         return t * (t + 1) / 2;
-                                      }) <= fuse_max();
+                                      }) <= fuse_max;
       outer_launch(s0, s_end, fused);
       if (!fused && !next_col)
         sched_launch(
@@ -993,13 +1010,35 @@ static void potrf_steps(const Batch& b, hipStream_t s) {
   }
 }
 
-void launch_potrf(const Batch& b, int prec, hipStream_t s) {
+static void potrf_class(const Batch& b, int prec, hipStream_t s) {
+  if (b.count <= 0) return;
+  const pgp_tune_entry_t te = tune_get(prec, tune_class_of(b.maxNp));
   if (prec == PGP_PREC_FP64)
-    potrf_steps<double, 0>(b, s);
+    potrf_steps<double, 0>(b, s, te);
   else if (prec == PGP_PREC_FP32)
-    potrf_steps<float, 0>(b, s);
+    potrf_steps<float, 0>(b, s, te);
   else
-    potrf_steps<float, 1>(b, s);
+    potrf_steps<float, 1>(b, s, te);
+}
+
+// The members of a batch are sorted by size (largest first), so the large
+// size class is a prefix: each class is factored with its own schedule.
+void launch_potrf(const Batch& b, int prec, hipStream_t s) {
+  int k = 0;
+  while (k < b.count && tune_class_of(b.np(k)) == 1) ++k;
+  if (k == 0 || k == b.count) {
+    potrf_class(b, prec, s);
+    return;
+  }
+  Batch big = b, small = b;
+  big.count = k;
+  small.dprobs = b.dprobs + k;
+  small.count = b.count - k;
+  small.maxNp = b.np(k);
+  if (b.hnp) small.hnp = b.hnp + k;
+  if (b.hm) small.hm = b.hm + k;
+  potrf_class(big, prec, s);
+  potrf_class(small, prec, s);
 }
 
 // The factorisation leaves DINV in place; kept as a no-op entry for callers

@@ -399,6 +399,135 @@ void launch_pred_cov(const Batch& b, int prec, int maxMp, double* cov, int64_t l
       });
 }
 
+// ---- FP64 correction of the predictive variance (factor stored in FP32) ----
+// The variance needs q_j = s_j^T B^-1 s_j, B = I + W^1/2 K W^1/2, s_j the
+// column j of S = W^1/2 Kfs.  The FP32 factor gives q_j = ||L^-1 s_j||^2 with
+// the factor's rounding error; instead, with z_j = (L L^T)^-1 s_j from the
+// two FP32 solves,
+//   q_j = 2 s_j^T z_j - z_j^T B z_j = s_j^T B^-1 s_j - e^T B e,  e = z_j - B^-1 s_j,
+// whose error is quadratic in the error of z_j.  One workgroup per 64 x 64
+// tile (I, J) of K U, U = W^1/2 Z, as an FP64 MFMA product over the Np
+// columns of K (FP64), then per entry z (2 s - z - W^1/2 (K U)) with s
+// recomputed in FP64 from the inputs, summed over the tile's 64 rows in a
+// fixed order into qpart[I][j].  Z (the predict buffer after both solves)
+// holds z in the factor's storage type.
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_pred_refine(Prob* const* probs, Sched sc) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int nrt = P.Np / TILE, nct = P.Mp / TILE;
+  if (bx >= nrt * nct) return;
+  const int ti = bx % nrt, tj = bx / nrt;
+  constexpr int PA = TILE + 1;
+  __shared__ double sa[TILE * PA], sb[TILE * PA];
+  __shared__ double red[2][TILE];
+  __shared__ KShared ks;
+  static_assert(TILE * PGP_MAX_DIM <= TILE * PA, "input rows fit the operand buffers");
+  const int D = P.D, M = P.M;
+  const size_t np = (size_t)P.Np;
+  const double* K = P.K;
+  const T* Z = (const T*)P.S;
+  const double* Ws = vslot(P, V_S);
+  const int kl = threadIdx.x & 63, cg = threadIdx.x >> 6;
+  doublex4 acc[2][2];
+  zero_acc<double>(acc);
+  // ra[q] = K(64 ti + kl, k0 + cg + 4q); rb[q] = U(k0 + kl, 64 tj + cg + 4q)
+  double ra[16], rb[16];
+  auto fetch = [&](int k0) {
+    const double w = Ws[k0 + kl];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cg + 4 * q;
+      ra[q] = K[(size_t)(ti * TILE + kl) + (size_t)(k0 + c) * np];
+      rb[q] = w * (double)Z[(size_t)(k0 + kl) + (size_t)(tj * TILE + c) * np];
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < P.Np; k0 += TILE) {
+    __syncthreads();  // the previous chunk's products are done with LDS
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = cg + 4 * q;
+      sa[c * PA + kl] = ra[q];  // A(m = kl, k = c), k-major
+      sb[kl * PA + c] = rb[q];  // B(k = kl, n = c), k-major
+    }
+    __syncthreads();
+    if (k0 + TILE < P.Np) fetch(k0 + TILE);
+    mma64_kmajor<double>(sa, PA, sb, PA, acc);
+  }
+  __syncthreads();  // operands are dead: the input rows take their place
+  double* xs = sa;
+  double* zs = sb;
+  stage_params(ks, P.inv_ls2, P.pi_over_per);
+  load_rows(xs, P.X, ti * TILE, P.N, D);
+  load_rows(zs, P.Xt, tj * TILE, M, D);
+  __syncthreads();
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double part[2] = {0.0, 0.0};
+#pragma unroll
+  for (int bn = 0; bn < 2; ++bn) {
+    const int col = acc_col(bn);
+    const int j = tj * TILE + col;
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = acc_row<double>(bm, r);
+        const int i = ti * TILE + row;
+        if (i < P.N && j < M) {
+          const double wi = Ws[i];
+          const double z = (double)Z[(size_t)i + (size_t)j * np];
+          const double sij = wi * kernel_value<double>(P.kern_kind, P.periodic, D, P.variance, ks,
+                                                       &xs[row * D], &zs[col * D]);
+          part[bn] += z * (2.0 * sij - z - wi * acc[bm][bn][r]);
+        }
+      }
+  }
+  // the 32 rows of this wave's half: lanes l, l + 16, l + 32, l + 48 hold
+  // the same column
+#pragma unroll
+  for (int bn = 0; bn < 2; ++bn) {
+    part[bn] += __shfl_xor(part[bn], 16);
+    part[bn] += __shfl_xor(part[bn], 32);
+  }
+  if (l < 16) {
+    red[w >> 1][32 * (w & 1) + l] = part[0];
+    red[w >> 1][32 * (w & 1) + 16 + l] = part[1];
+  }
+  __syncthreads();
+  if (threadIdx.x < TILE)
+    P.qpart[(size_t)ti * P.Mp + tj * TILE + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
+}
+
+void launch_pred_refine(const Batch& b, int prec, int maxM, hipStream_t s) {
+  const bool on = timer_on(TC_REFINE);
+  sched_launch(
+      b,
+      [&](int m) {
+        const int mt = (b.mcount(m, maxM) + TILE - 1) / TILE;
+        return (b.np(m) / TILE) * mt;
+      },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        double fl = 0.0, by = 0.0;
+        if (on) {
+          // algorithmic work: the K U product, 2 Np^2 Mp flops; bytes: K read once
+          // per column tile, Z read twice
+          const int off = (int)(dp - b.dprobs);
+          for (int m = off; m < off + sc.count; ++m) {
+            const double np = b.np(m), mp = (b.mcount(m, maxM) + TILE - 1) / TILE * TILE;
+            fl += 2.0 * np * np * mp;
+            by += np * np * 8.0 * (mp / TILE) + 2.0 * np * mp * (prec == PGP_PREC_FP64 ? 8 : 4);
+          }
+          timer_begin(TC_REFINE, s);
+        }
+        if (prec == PGP_PREC_FP64)
+          hipLaunchKernelGGL(k_pred_refine<double>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+        else
+          hipLaunchKernelGGL(k_pred_refine<float>, dim3(n), dim3(NTHREADS), 0, s, dp, sc);
+        if (on) timer_end(TC_REFINE, s, fl, by);
+      });
+}
+
 // ---- raw Gram (kernel-level ABI) ----
 template <typename TC, typename TO>
 __global__ __launch_bounds__(NTHREADS) void k_gram_raw(KParams kp, const double* X, int N,

@@ -337,4 +337,77 @@ int launch_probit_predictive(const pgp_likelihood_t* lik, const double* mean, co
   return 0;
 }
 
+// Ordinal test metrics (examples/classification.py:325-341) from class
+// probabilities P (M x J, row-major, as k_probit_predictive writes them) and
+// test labels y: y_pred_i = argmax_j P_ij (the first maximum, as argmax),
+// and -- the reference indexes P[:, y_test] -- the M x M matrix
+// L_ij = P(i, y_j), whose log-sum is its log predictive probability.  One
+// workgroup per row i: its part of that sum (fixed-order reduction), the
+// prediction, |y_pred_i - y_i| and [y_pred_i != y_i] into the row workspace.
+// A label outside 0..J-1 reads no memory and gives NaN.
+__global__ __launch_bounds__(NTHREADS) void k_metrics_rows(const double* P, int J,
+                                                           const double* y, int64_t M,
+                                                           double* plik, int32_t* y_pred,
+                                                           double* rows) {
+  __shared__ double red[NTHREADS];
+  const int64_t i = blockIdx.x;
+  const double* Pi = P + i * J;
+  double acc = 0.0;
+  for (int64_t j = threadIdx.x; j < M; j += NTHREADS) {
+    const int c = (int)y[j];
+    const double v = (c >= 0 && c < J && (double)c == y[j]) ? Pi[c] : NAN;
+    if (plik) plik[i * M + j] = v;
+    acc += log(v);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = NTHREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int best = 0;
+    for (int c = 1; c < J; ++c)
+      if (Pi[c] > Pi[best]) best = c;
+    y_pred[i] = best;
+    rows[i] = red[0];
+    rows[M + i] = fabs((double)best - y[i]);
+    rows[2 * M + i] = ((double)best != y[i]) ? 1.0 : 0.0;
+  }
+}
+
+// Column sums of the row workspace in a fixed order (one workgroup):
+// metrics = [incorrect, correct, mean absolute error, log predictive
+// probability, mean zero-one error].
+__global__ __launch_bounds__(NTHREADS) void k_metrics_final(const double* rows, int64_t M,
+                                                            double* metrics) {
+  __shared__ double red[3][NTHREADS];
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = threadIdx.x; i < M; i += NTHREADS)
+    for (int q = 0; q < 3; ++q) a[q] += rows[q * M + i];
+  for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int h = NTHREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n = (double)M;
+    metrics[0] = red[2][0];
+    metrics[1] = n - red[2][0];
+    metrics[2] = red[1][0] / n;
+    metrics[3] = red[0][0];
+    metrics[4] = red[2][0] / n;
+  }
+}
+
+int launch_ordinal_metrics(const double* P, int J, const double* y, int64_t M, int32_t* y_pred,
+                           double* plik, double* metrics, double* rows, hipStream_t s) {
+  hipLaunchKernelGGL(k_metrics_rows, dim3((unsigned)M), dim3(NTHREADS), 0, s, P, J, y, M, plik,
+                     y_pred, rows);
+  hipLaunchKernelGGL(k_metrics_final, dim3(1), dim3(NTHREADS), 0, s, rows, M, metrics);
+  return 0;
+}
+
 }  // namespace pgp

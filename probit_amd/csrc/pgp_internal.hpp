@@ -97,6 +97,9 @@ struct Prob {
   double* symv;
   // wavefront TRSM: one progress word per right-hand-side column tile (4)
   uint64_t* progress_ct;
+  // predict with the FP64 variance correction: (Np/64) x Mp partial sums
+  // (NULL: the variance from the column norms of L^-1 S)
+  double* qpart;
 };
 
 __host__ __device__ inline double* vslot(const Prob& p, int v) {
@@ -242,7 +245,8 @@ __device__ __forceinline__ int wait_progress(uint64_t* progress, uint32_t epoch,
 //   TC_TRSV  single-launch triangular solves
 //   TC_GRAM  Gram assembly
 //   TC_TRSM  multi-right-hand-side solves
-enum TimerClass { TC_SYRK = 0, TC_PANEL, TC_TRSV, TC_GRAM, TC_TRSM, TC_COUNT };
+//   TC_REFINE FP64 variance correction of predict (K U product)
+enum TimerClass { TC_SYRK = 0, TC_PANEL, TC_TRSV, TC_GRAM, TC_TRSM, TC_REFINE, TC_COUNT };
 void timer_begin(int cls, hipStream_t s);
 void timer_end(int cls, hipStream_t s, double algo_flops, double algo_bytes);
 bool timer_on(int cls);
@@ -253,6 +257,7 @@ void launch_gram_self(const Batch& b, hipStream_t s);
 // launch_symv_reduce)
 void launch_form_B(const Batch& b, int prec_vb, hipStream_t s, int rs = -1);
 void launch_cross_prep(const Batch& b, int prec, int maxM, hipStream_t s);
+void launch_pred_refine(const Batch& b, int prec, int maxM, hipStream_t s);
 int launch_gram_raw(const pgp_kernel_t* k, int dtype, const double* X, int64_t N,
                     const double* Z, int64_t M, void* out, int64_t ld, hipStream_t s);
 
@@ -264,9 +269,14 @@ int launch_lik_raw(const pgp_likelihood_t* lik, const double* f, const double* y
 void launch_lik_grad_terms(const Batch& b, double* out, hipStream_t s);
 int launch_probit_predictive(const pgp_likelihood_t* lik, const double* mean, const double* var,
                              int64_t M, double* out, hipStream_t s);
+int launch_ordinal_metrics(const double* P, int J, const double* y, int64_t M, int32_t* y_pred,
+                           double* plik, double* metrics, double* rows, hipStream_t s);
 
 // chol.hip
 void launch_potrf(const Batch& b, int prec, hipStream_t s);
+int tune_class_of(int np);
+pgp_tune_entry_t tune_get(int prec, int cls);
+void tune_set(int prec, int cls, const pgp_tune_entry_t& v);
 void launch_dinv(const Batch& b, int prec, hipStream_t s);
 int launch_potrf_raw(int prec, void* A, int64_t N, int64_t ld, double* dinv, double* logdiag,
                      int32_t* info, hipStream_t s);
@@ -286,6 +296,7 @@ int launch_dinv_raw(int prec, const void* A, int64_t N, int64_t ld, double* dinv
 // solve.hip
 void launch_trsv(const Batch& b, int prec, int trans, int in, int out, hipStream_t s);
 void launch_trsm(const Batch& b, int prec, int maxMp, int lower_rhs, hipStream_t s);
+void launch_trsm_t(const Batch& b, int prec, int maxMp, hipStream_t s);
 int launch_trsv_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const double* dinv,
                          double* b, double* x, int trans, hipStream_t s);
 int launch_trsm_raw_dinv(int prec, const void* L, int64_t N, int64_t ld, const double* dinv,

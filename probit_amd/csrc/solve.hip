@@ -585,6 +585,159 @@ __global__ __launch_bounds__(NTHREADS) void k_trsm_rows(Prob* const* probs, Sche
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Transposed wavefront TRSM: X <- L^-T X in place on the predict buffer S,
+// for the column tiles [ct0, ct0 + nct) (nct <= WAVE_NCT), all members in
+// one launch.  Step t of a member is its block row i = nb - 1 - t (the sweep
+// runs from the last block row up): workgroup (t, member, ct) accumulates
+// L_ji^T X_j[:, ct] for every j > i as soon as step nb - 1 - j publishes,
+// then forms X_i = DINV_i^T (X_i - acc) and publishes step t.  Operands as in
+// k_trsm_rows with both L_ji and DINV_i transposed on their way into LDS
+// (rows of the stored tile are the k index, read coalesced along k).
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_trsm_rows_t(Prob* const* probs, Sched rows, int nct,
+                                                          int ct0, uint32_t* ticket,
+                                                          uint32_t epoch) {
+  constexpr int PA = TILE + 1;  // A operand, written row by row
+  constexpr int PB = TILE + 1;  // B operand, written transposed
+  __shared__ T As[TILE * PA];
+  __shared__ T Bs[TILE * PB];
+  __shared__ int s_val;
+  typedef typename Acc<T>::type acc_t;
+  const int tk = take_ticket(ticket, &s_val);
+  if ((unsigned)tk >= gridDim.x) {  // counter not zero at launch: no block to take
+    flag_members_stalled(probs, (rows.pre[1] - rows.pre[0]) / nct);  // step 0: every member
+    return;
+  }
+  int local;
+  const int t = sched_member(rows, tk, local);
+  const int m = local / nct, lct = local - m * nct;
+  const Prob& P = *probs[m];
+  const int nb = P.Np / TILE, ct = ct0 + lct;
+  if (ct >= P.Mp / TILE || t >= nb) return;
+  const int i = nb - 1 - t;
+  uint64_t* prog = P.progress_ct + lct;
+  const auto L = gbl((const T*)P.L);
+  T* S = (T*)P.S;
+  const int64_t ld = P.Np;
+  const int64_t ri = (int64_t)i * TILE, n0 = (int64_t)ct * TILE;
+  const int r = threadIdx.x & 63, cb = threadIdx.x >> 6;
+  T dv[16], la[16], sv[2][2][4];
+  {
+    const auto D = gbl(P.dinv + (size_t)i * TILE * TILE);
+    const auto Sg = gbl((const T*)P.S);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dv[q] = (T)D[r + (size_t)(cb + 4 * q) * TILE];
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          sv[bm][bn][q] =
+              Sg[(size_t)(ri + acc_row<T>(bm, q)) + (size_t)(n0 + acc_col(bn)) * ld];
+  }
+  // la[q] = L_ji(r, cb + 4q) = L(64 j + r, 64 i + cb + 4q): A(m, k) = L_ji(k, m)
+  auto fetch_l = [&](int j) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      la[q] = L[(size_t)((int64_t)j * TILE + r) + (size_t)(ri + cb + 4 * q) * ld];
+  };
+  if (t > 0) fetch_l(nb - 1);
+  acc_t acc[2][2];
+  zero_acc<T>(acc);
+  int done = 0, ready = 0;
+  while (done < t) {
+    if (done >= ready) {
+      __syncthreads();  // everyone has read the previous s_val
+      if (threadIdx.x == 0) s_val = wait_progress(prog, epoch, done + 1, P.info);
+      __syncthreads();
+      ready = min(__builtin_amdgcn_readfirstlane(s_val), t);
+    }
+    for (; done < ready; ++done) {
+      const int64_t rj = (int64_t)(nb - 1 - done) * TILE;
+      __syncthreads();  // the previous product is done with LDS
+#pragma unroll
+      for (int q = 0; q < 16; ++q) As[r * PA + cb + 4 * q] = la[q];
+      // B[kk][n] = X_j(kk, n), published in this launch: coherent loads
+#pragma unroll 4
+      for (int q = 0; q < 16; ++q) {
+        const int c = cb + 4 * q;
+        Bs[r * PB + c] = __hip_atomic_load(&S[(size_t)(rj + r) + (size_t)(n0 + c) * ld],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (done + 1 < t) fetch_l(nb - 2 - done);
+      __syncthreads();
+      mma64_kmajor<T>(As, PA, Bs, PB, acc);
+    }
+  }
+  // X_i = DINV_i^T (X_i - acc)
+  __syncthreads();
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = acc_row<T>(bm, q), col = acc_col(bn);
+        Bs[row * PB + col] = sv[bm][bn][q] - acc[bm][bn][q];
+      }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) As[r * PA + cb + 4 * q] = dv[q];  // A(m, k) = DINV_i(k, m)
+  __syncthreads();
+  zero_acc<T>(acc);
+  mma64_kmajor<T>(As, PA, Bs, PB, acc);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        __hip_atomic_store(&S[(size_t)(ri + acc_row<T>(bm, q)) + (size_t)(n0 + acc_col(bn)) * ld],
+                           (T)acc[bm][bn][q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  publish_wait();  // every wave's X_i stores complete ...
+  __syncthreads();
+  if (threadIdx.x == 0)  // ... before step t + 1 may read them
+    __hip_atomic_store(prog, ((uint64_t)epoch << 32) | (uint64_t)(t + 1), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// S <- L^-T S for every member's Mp right-hand sides (Mp <= maxMp): one
+// wavefront launch per group of WAVE_NCT column tiles.
+void launch_trsm_t(const Batch& bt, int prec, int maxMp, hipStream_t s) {
+  const int nb = bt.maxNp / TILE;
+  const int mt = maxMp / TILE;
+  for (int ct0 = 0; ct0 < mt; ct0 += WAVE_NCT) {
+    const int nct = std::min(WAVE_NCT, mt - ct0);
+    Sched rows;
+    rows.count = nb;
+    int tot = 0;
+    for (int t = 0; t < nb; ++t) {
+      rows.pre[t] = tot;
+      for (int m = 0; m < bt.count; ++m)
+        if (bt.np(m) / TILE > t) tot += nct;
+    }
+    rows.pre[nb] = tot;
+    if (tot == 0) return;
+    const bool on = timer_on(TC_TRSM);
+    double fl = 0.0;
+    if (on) {
+      for (int m = 0; m < bt.count; ++m) {
+        const double np = bt.np(m);
+        fl += np * np * (double)nct * TILE;
+      }
+      timer_begin(TC_TRSM, s);
+    }
+    const uint32_t ep = next_epoch();
+    if (prec == PGP_PREC_FP64)
+      hipLaunchKernelGGL(k_trsm_rows_t<double>, dim3(tot), dim3(NTHREADS), 0, s, bt.dprobs, rows,
+                         nct, ct0, bt.ticket, ep);
+    else
+      hipLaunchKernelGGL(k_trsm_rows_t<float>, dim3(tot), dim3(NTHREADS), 0, s, bt.dprobs, rows,
+                         nct, ct0, bt.ticket, ep);
+    if (on) timer_end(TC_TRSM, s, fl, 0.0);
+  }
+}
+
 static bool wavefront_trsm() {
   static const bool v = [] {
     const char* e = getenv("PGP_TRSM_WAVEFRONT");

@@ -381,21 +381,28 @@ __global__ __launch_bounds__(NTHREADS) void k_predict_final(Prob* const* probs, 
   const int j = bx * 4 + (threadIdx.x >> 6);
   if (j >= P.M) return;
   const int lane = threadIdx.x & 63;
-  const T* S = (const T*)P.S + (size_t)j * P.Np;
-  double q0 = 0.0, q1 = 0.0;
-  int i = lane;
-  for (; i + 64 < P.Np; i += 128) {
-    const double v0 = (double)S[i], v1 = (double)S[i + 64];
-    q0 += v0 * v0;
-    q1 += v1 * v1;
-  }
-  for (; i < P.Np; i += 64) {
-    const double v = (double)S[i];
-    q0 += v * v;
-  }
-  const double q = wave_sum(q0 + q1);
-  double mu = 0.0;
   const int nrt = P.Np / TILE;
+  double q;
+  if (P.qpart) {  // the FP64-corrected quadratic forms, summed over row tiles
+    double qa = 0.0;
+    for (int rt = lane; rt < nrt; rt += 64) qa += P.qpart[(size_t)rt * P.Mp + j];
+    q = wave_sum(qa);
+  } else {
+    const T* S = (const T*)P.S + (size_t)j * P.Np;
+    double q0 = 0.0, q1 = 0.0;
+    int i = lane;
+    for (; i + 64 < P.Np; i += 128) {
+      const double v0 = (double)S[i], v1 = (double)S[i + 64];
+      q0 += v0 * v0;
+      q1 += v1 * v1;
+    }
+    for (; i < P.Np; i += 64) {
+      const double v = (double)S[i];
+      q0 += v * v;
+    }
+    q = wave_sum(q0 + q1);
+  }
+  double mu = 0.0;
   for (int rt = lane; rt < nrt; rt += 64) mu += P.mupart[(size_t)rt * P.Mp + j];
   mu = wave_sum(mu);
   if (lane == 0) {

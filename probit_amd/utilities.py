@@ -28,6 +28,7 @@ __all__ = [
     "grad_log_probit_likelihood",
     "hessian_log_probit_likelihood",
     "probit_predictive_distributions",
+    "calculate_metrics",
     "check_data",
     "check_cutpoints",
     "ndtr",
@@ -210,6 +211,45 @@ def probit_predictive_distributions(likelihood_parameters, posterior_mean, poste
                                    out.data_ptr(), _lib.stream_handle(dev))
     _lib.check(rc, "pgp_probit_predictive")
     return out
+
+
+def calculate_metrics(y_test, predictive_distributions, verbose=True):
+    """Test metrics of the reference's classification example
+    (examples/classification.py:325-341) on the GPU (pgp_ordinal_metrics):
+    returns (mean_zero_one, mean_absolute_error, log_predictive_probability,
+    predictive_likelihood) and, with ``verbose``, prints the reference's lines.
+
+    As in the reference, ``predictive_likelihood`` is
+    ``predictive_distributions[:, y_test]`` -- an (M, M) matrix, column j
+    holding every test point's probability of class y_test[j] -- and the log
+    predictive probability sums its log over all M * M entries; the numbers
+    are the reference's, not the per-point log likelihood."""
+    lib = _lib.lib()
+    P = torch.as_tensor(predictive_distributions, dtype=torch.float64)
+    dev = P.device if P.is_cuda else torch.device("cuda")
+    P = P.to(dev).contiguous()
+    if P.dim() != 2:
+        raise ValueError("predictive_distributions must be (M, J)")
+    M, J = P.shape
+    y = torch.as_tensor(y_test).reshape(-1).to(device=dev, dtype=torch.float64).contiguous()
+    if y.numel() != M:
+        raise ValueError("y_test and predictive_distributions disagree on M")
+    y_pred = torch.empty(M, dtype=torch.int32, device=dev)
+    plik = torch.empty((M, M), dtype=torch.float64, device=dev)
+    metrics = torch.empty(5, dtype=torch.float64, device=dev)
+    ws = torch.empty(3 * M, dtype=torch.float64, device=dev)
+    rc = lib.pgp_ordinal_metrics(P.data_ptr(), J, y.data_ptr(), M, y_pred.data_ptr(),
+                                 plik.data_ptr(), metrics.data_ptr(), ws.data_ptr(),
+                                 _lib.stream_handle(dev))
+    _lib.check(rc, "pgp_ordinal_metrics")
+    n_bad, n_good, mae, lpp, zero_one = metrics.cpu().tolist()
+    if verbose:
+        print(int(n_bad), "sum incorrect")
+        print(int(n_good), "sum correct")
+        print("mean_absolute_error={:.2f}".format(mae))
+        print("log_pred_probability={:.2f}".format(lpp))
+        print("mean_zero_one_error={:.2f}".format(zero_one))
+    return zero_one, mae, lpp, plik
 
 
 # --------------------------------------------------------------------------

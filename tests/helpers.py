@@ -58,3 +58,25 @@ def rel_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b))))
+
+
+def oracle_value_and_grad(X, y, kind, lik):
+    """value_and_grad for the oracle at the FP64 Laplace fixed point of each
+    parameter set ((lengthscale, variance), likelihood parameters), isotropic
+    `kind` kernel: objective_LA and its closed-form total derivative, shaped
+    like the parameters (the quantity jax.value_and_grad gives through
+    fixed_point_layer, probit/approximators.py:132-134)."""
+    lk = lik_kind(lik)
+
+    def fn(params):
+        (ls, var), lp = params
+        spec = oracle_spec(kind, float(var), float(ls), X.shape[1])
+        K = O.gram(spec, X)
+        w = O.laplace_weight_converged(K, y, lk, lp)
+        val = O.objective_LA(spec, X, y, lk, lp, w)
+        g = O.laplace_objective_gradient(spec, X, y, lk, lp, w)
+        g_ls = sum(g[f"lengthscale{d}"] for d in range(X.shape[1]))
+        g_lik = (g["noise_std"],) if lik == "gaussian" else (g["noise_std"], g["cutpoints"])
+        return float(val), ((g_ls, g["variance"]), g_lik)
+
+    return fn

@@ -170,6 +170,15 @@ CASES = [
         None, _gauss(L), FAKE, FAKE, 0, FAKE, 0, None), -1, "kernel is NULL"),
     ("vb_grad: likelihood NULL", lambda L, lib: lib.pgp_vb_grad(
         _kernel(L), None, FAKE, FAKE, 0, FAKE, 0, None), -2, "likelihood is NULL"),
+    ("tune: precision", lambda L, lib: lib.pgp_tune_get(3, 0, L.TuneEntryT()), -1,
+     "unknown precision"),
+    ("tune: class", lambda L, lib: lib.pgp_tune_get(1, 2, L.TuneEntryT()), -2, "size class"),
+    ("tune: superstep 0", lambda L, lib: lib.pgp_tune_set(1, 0, L.TuneEntryT(0, 512, 512, 0)),
+     -3, "superstep"),
+    ("tune: superstep 17", lambda L, lib: lib.pgp_tune_set(1, 0, L.TuneEntryT(17, 512, 512, 0)),
+     -3, "superstep"),
+    ("tune: negative column_max", lambda L, lib: lib.pgp_tune_set(
+        1, 1, L.TuneEntryT(8, -1, 512, 0)), -3, "column_max"),
     ("status mapping: result NULL", lambda L, lib: lib.pgp_fit_result_apply_info(None, 1), -1,
      "result is NULL"),
     ("gram: kernel NULL", lambda L, lib: lib.pgp_gram(None, 0, FAKE, 10, FAKE, 10, FAKE, 10, None),
@@ -294,3 +303,18 @@ def test_fit_status_mapping(word, prior, bits, nan):
     assert math.isnan(r.objective) == nan
     if not nan:
         assert r.objective == 12.5
+
+
+def test_tune_table_round_trips_on_the_host():
+    """The schedule table needs no GPU: set, get back, restore."""
+    L, lib = _lib()
+    old = L.TuneEntryT()
+    assert lib.pgp_tune_get(2, 1, old) == 0
+    assert 1 <= old.superstep <= 16
+    assert lib.pgp_tune_set(2, 1, L.TuneEntryT(5, 100, 0, 77)) == 0
+    got = L.TuneEntryT()
+    assert lib.pgp_tune_get(2, 1, got) == 0
+    assert (got.superstep, got.column_max, got.fuse_max, got.reserved) == (5, 100, 0, 0)
+    other = L.TuneEntryT()
+    assert lib.pgp_tune_get(2, 0, other) == 0 and other.superstep == old.superstep
+    assert lib.pgp_tune_set(2, 1, old) == 0

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import json
 import math
+import os
 import time
 
 import numpy as np
@@ -128,6 +129,21 @@ def test_c3_single_large_fit_precision_variants():
         torch.cuda.empty_cache()
     w64, m64, v64, o64 = res["fp64"]
     assert out["fp64_status"] == 0
+    # oracle-side properties at full size, as for the c2 configuration: the
+    # reference's fixed-point map evaluated in FP64 at the FP64 weight, the
+    # predictive mean as Kfs^T a, 0 <= var <= Kss
+    K = _gram(p, p.X, p.X)
+    out["fp64_fixed_point_residual"] = _residual(p, K, w64)
+    del K
+    torch.cuda.empty_cache()
+    assert out["fp64_fixed_point_residual"] < 1e-4
+    Kfs = _gram(p, p.X, p.Xtest)
+    m_ref = torch.mv(Kfs.t().contiguous(), torch.as_tensor(w64, device="cuda")).cpu().numpy()
+    del Kfs
+    torch.cuda.empty_cache()
+    out["fp64_mean_vs_Kfs_a"] = _rel(m64, m_ref)
+    assert out["fp64_mean_vs_Kfs_a"] < 1e-12
+    assert np.all(v64 > -1e-9) and np.all(v64 <= 1.0 + 1e-12)
     for mode in ("fp32", "bf16"):
         w, m, v, o = res[mode]
         out[f"{mode}_weight_rel"] = _rel(w, w64)
@@ -198,25 +214,68 @@ def test_c4_evidence_gradient_n8192(lik):
         assert rel[k] < 1e-6, (k, got[k], rich[k])
 
 
-def test_c4_lbfgsb_50_steps():
-    """50 L-BFGS-B iterations of the hyper-parameters end to end (objective +
-    adjoint on the GPU each evaluation), probit J=3, N=2048."""
+def _flat(params):
+    (ls, var), (s, c) = params
+    return np.array([float(np.sum(ls)), float(var), float(s), *np.asarray(c)[1:-1]])
+
+
+def test_c4_lbfgsb_matches_oracle_driven_optimiser_n1024():
+    """The hyper-parameter loop (at most 50 L-BFGS-B iterations,
+    examples/classification.py:454) driven by the HIP objective + adjoint and
+    by the oracle's objective_LA + closed-form gradient, through the same
+    SciPy wrapper from the same start: the same evaluations (values to 1e-8
+    relative) and the same optimum."""
     from probit_amd import minimise_l_bfgs_b
-    p = DG.evidence_gradient(seed=3, N=2048, lik="probit")
-    gp, params = _gp(p, 1e-8, "fp64")
+    from tests.helpers import oracle_value_and_grad
+    p = DG.evidence_gradient(seed=3, N=1024, lik="probit")
+    gp, params = _gp(p, 1e-10, "fp64")
     (ls, var), (noise, cut) = params
     start = ((0.5 * ls, 0.7 * var), (1.3 * noise, cut))
-    t0 = time.perf_counter()
     opt, hist = minimise_l_bfgs_b(gp.value_and_grad(), start, iters=50)
+    o_opt, o_hist = minimise_l_bfgs_b(oracle_value_and_grad(p.X, p.y, "EQ", "probit"), start,
+                                      iters=50)
+    print(json.dumps({"config": "c4-lbfgsb-vs-oracle", "N": 1024, "evaluations": len(hist),
+                      "oracle_evaluations": len(o_hist), "first": hist[0], "best": min(hist),
+                      "oracle_best": min(o_hist)}))
+    assert len(hist) == len(o_hist)
+    assert np.allclose(hist, o_hist, rtol=1e-8, atol=0)
+    assert np.allclose(_flat(opt), _flat(o_opt), rtol=1e-6, atol=1e-8)
+    assert np.all(np.diff(opt[1][1]) > 0)
+
+
+def test_c4_lbfgsb_50_steps_n8192_matches_oracle_trajectory():
+    """The evidence-gradient configuration's hyper-parameter loop at full size
+    (probit J = 3, N = 8192, D = 4, at most 50 L-BFGS-B iterations) on the
+    GPU, against the trajectory the oracle-driven loop wrote
+    (tests/golden/make_c4_lbfgsb.py): every evaluation's parameters and
+    objective value, and the optimum."""
+    from probit_amd import minimise_l_bfgs_b
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c4_lbfgsb_probit_n8192.npz"))
+    p = DG.evidence_gradient(seed=3, N=8192, lik="probit")
+    gp, params = _gp(p, 1e-10, "fp64")
+    (ls, var), (noise, cut) = params
+    start = ((0.5 * ls, 0.7 * var), (1.3 * noise, cut))
+    assert np.allclose(_flat(start), g["start"], rtol=1e-15, atol=0)
+    evals = []
+    f = gp.value_and_grad()
+
+    def logged(q):
+        v, gr = f(q)
+        evals.append([*_flat(q), v])
+        return v, gr
+
+    t0 = time.perf_counter()
+    opt, hist = minimise_l_bfgs_b(logged, start, iters=50)
     secs = time.perf_counter() - t0
-    print(json.dumps({"config": "c4-lbfgsb", "N": 2048, "evaluations": len(hist),
-                      "seconds": round(secs, 2), "first": hist[0], "best": min(hist),
-                      "opt": [[float(np.sum(opt[0][0])), float(opt[0][1])],
-                              [float(opt[1][0]), opt[1][1][1:-1].tolist()]]}))
-    assert min(hist) < hist[0]
-    assert np.all(np.isfinite(hist[-5:]))
-    cut_opt = opt[1][1]
-    assert np.all(np.diff(cut_opt) > 0)
+    ev = np.array(evals)
+    print(json.dumps({"config": "c4-lbfgsb", "N": 8192, "evaluations": len(hist),
+                      "oracle_evaluations": int(g["history"].size), "seconds": round(secs, 2),
+                      "first": hist[0], "best": min(hist), "oracle_best": float(g["history"].min()),
+                      "opt": _flat(opt).tolist(), "oracle_opt": g["opt"].tolist()}))
+    assert len(hist) == g["history"].size
+    assert np.allclose(ev, g["evals"], rtol=1e-7, atol=1e-10)
+    assert np.allclose(_flat(opt), g["opt"], rtol=1e-6, atol=1e-8)
+    assert np.all(np.diff(opt[1][1]) > 0)
 
 
 def test_prior_sample_c2_latent_n12288():

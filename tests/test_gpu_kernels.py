@@ -135,6 +135,37 @@ def test_closed_form_probit_every_regime_matches_oracle(single):
         assert ok.all(), (name, single, f[bad[:5]], y[bad[:5]], g[bad[:5]], r[bad[:5]])
 
 
+def test_ordinal_metrics_match_oracle():
+    """calculate_metrics (examples/classification.py:325-341) on the GPU: the
+    argmax, the M x M P[:, y_test] matrix and the five numbers, against the
+    oracle; a tie takes the first class, as argmax does."""
+    torch, _lib, lib = _setup()
+    import probit_amd as P
+    rng = np.random.default_rng(9)
+    M, J = 700, 4
+    Pm = rng.dirichlet(np.ones(J), size=M)
+    Pm[5] = [0.4, 0.4, 0.1, 0.1]  # tie: class 0
+    y = rng.integers(0, J, size=M)
+    zo, mae, lpp, plik = P.calculate_metrics(y, torch.as_tensor(Pm, device="cuda"),
+                                             verbose=False)
+    r_zo, r_mae, r_lpp, r_plik, r_pred = O.calculate_metrics(y, Pm)
+    assert np.array_equal(plik.cpu().numpy(), r_plik)
+    assert zo == r_zo and mae == r_mae
+    assert abs(lpp - r_lpp) <= 1e-12 * abs(r_lpp)
+    yp = torch.empty(M, dtype=torch.int32, device="cuda")
+    met = torch.empty(5, dtype=torch.float64, device="cuda")
+    ws = torch.empty(3 * M, dtype=torch.float64, device="cuda")
+    Pd = torch.as_tensor(Pm, device="cuda")
+    yd = torch.as_tensor(y.astype(np.float64), device="cuda")
+    _lib.check(lib.pgp_ordinal_metrics(Pd.data_ptr(), J, yd.data_ptr(), M, yp.data_ptr(), None,
+                                       met.data_ptr(), ws.data_ptr(), _lib.stream_handle()),
+               "pgp_ordinal_metrics")
+    assert np.array_equal(yp.cpu().numpy(), r_pred)
+    assert yp.cpu().numpy()[5] == 0
+    m = met.cpu().numpy()
+    assert m[0] == np.sum(r_pred != y) and m[1] == np.sum(r_pred == y)
+
+
 def _spd(rng, N):
     X = rng.normal(size=(N, 2))
     K = O.gram(O.KernelSpec.make(O.EQ, 1.0, 0.7, 2), X)

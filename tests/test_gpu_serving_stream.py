@@ -82,6 +82,9 @@ def test_serving_stream_slice_matches_oracle(stream_slice, precision):
     eng.close()
     assert sorted(done) == sorted(r.rid for r in reqs)
     tol = 1e-9 if precision == "fp64" else 1e-5
+    # variance: FP32 / BF16 factors get the FP64 correction of the quadratic
+    # form (its error is quadratic in the FP32 solve's)
+    vtol = tol
     worst = {"mean": 0.0, "var": 0.0, "extra_iterations": 0}
     for r in reqs:
         c = done[r.rid]
@@ -97,7 +100,7 @@ def test_serving_stream_slice_matches_oracle(stream_slice, precision):
         em = np.max(np.abs(c.mean.cpu().numpy() - m_ref)) / np.max(np.abs(m_ref))
         ev = np.max(np.abs(c.var.cpu().numpy() - v_ref)) / np.max(np.abs(v_ref))
         ew = np.max(np.abs(c.weight.cpu().numpy() - w_ref)) / np.max(np.abs(w_ref))
-        assert em <= tol and ev <= tol and ew <= tol, \
+        assert em <= tol and ev <= vtol and ew <= tol, \
             (r.rid, r.problem.N, r.problem.D, r.problem.lik, em, ev, ew)
         worst["mean"] = max(worst["mean"], em)
         worst["var"] = max(worst["var"], ev)

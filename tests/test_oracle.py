@@ -373,3 +373,17 @@ def test_prior_sample_of_uncorrelated_points_scales_z():
     z = np.random.default_rng(6).standard_normal(5)
     f = O.prior_sample(spec, X, z, 1e-3)
     assert np.allclose(f, np.sqrt(2.0 + 1e-3) * z, rtol=1e-12, atol=0.0)
+
+
+def test_calculate_metrics_hand_example():
+    """examples/classification.py:325-341 on a 3-point, 2-class example worked
+    by hand: argmax rows [1, 0, 1]; P[:, y] with y = [1, 1, 0] is the 3 x 3
+    matrix of columns P[:, 1], P[:, 1], P[:, 0]."""
+    P = np.array([[0.2, 0.8], [0.7, 0.3], [0.4, 0.6]])
+    y = np.array([1, 1, 0])
+    zo, mae, lpp, plik, y_pred = O.calculate_metrics(y, P)
+    assert list(y_pred) == [1, 0, 1]
+    assert zo == 2 / 3 and mae == 2 / 3
+    expect = np.array([[0.8, 0.8, 0.2], [0.3, 0.3, 0.7], [0.6, 0.6, 0.4]])
+    assert np.array_equal(plik, expect)
+    assert np.isclose(lpp, 2 * np.log(0.8 * 0.3 * 0.6) + np.log(0.2 * 0.7 * 0.4), rtol=1e-14)
