@@ -95,8 +95,10 @@ def test_serving_stream_slice_matches_oracle(stream_slice, precision):
             assert c.iterations == it_ref, (r.rid, c.iterations, it_ref)
         else:
             # an FP32 / BF16 factor with the FP64 residual is an inexact Newton
-            # to the same fixed point: the same count or a few more steps
-            assert it_ref <= c.iterations <= it_ref + 4, (r.rid, c.iterations, it_ref)
+            # to the same fixed point: the same count or a few more steps (BF16
+            # updates contract more slowly until a stall escalates them)
+            extra = 4 if precision == "fp32" else 8
+            assert it_ref <= c.iterations <= it_ref + extra, (r.rid, c.iterations, it_ref)
         em = np.max(np.abs(c.mean.cpu().numpy() - m_ref)) / np.max(np.abs(m_ref))
         ev = np.max(np.abs(c.var.cpu().numpy() - v_ref)) / np.max(np.abs(v_ref))
         ew = np.max(np.abs(c.weight.cpu().numpy() - w_ref)) / np.max(np.abs(w_ref))
