@@ -34,7 +34,7 @@ from . import _lib
 __all__ = ["CANDIDATES", "PACKAGE_TABLE", "device_arch", "get", "set_entry", "table", "save",
            "load", "tune"]
 
-CANDIDATES = {"superstep": (4, 6, 8, 12, 16), "column_max": (0, 256, 512, 1024, 4096),
+CANDIDATES = {"superstep": (2, 3, 4, 6, 8, 12, 16), "column_max": (0, 256, 512, 1024, 4096),
               "fuse_max": (0, 512)}
 PACKAGE_TABLE = Path(__file__).resolve().parent / "tune_table.json"
 _FIELDS = ("superstep", "column_max", "fuse_max")
