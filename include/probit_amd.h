@@ -274,6 +274,24 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
                     pgp_fit_result_t* result, void* workspace,
                     int64_t workspace_bytes, void* stream);
 
+/* pgp_laplace_fit without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a static device array inside the library, one per GPU, held
+ * until the process ends (pgp_noalloc_scratch_bytes() bytes: the workspace
+ * of N = PGP_NOALLOC_MAX_NP at any precision); larger N gives a negative
+ * status.  The call allocates nothing: no hipMalloc / hipFree, no
+ * allocating library routine.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race,
+ * and the resident factor a call leaves is overwritten by the next call (no
+ * predict / objective from it: use pgp_laplace_fit for those).  Outputs,
+ * status and result as pgp_laplace_fit. */
+#define PGP_NOALLOC_MAX_NP 1024
+int64_t pgp_noalloc_scratch_bytes(void);
+int pgp_laplace_fit_noalloc(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+                            const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
+                            pgp_fit_result_t* result, void* stream);
+
 /* Re-form and factor B from a caller-given (weight, precision) pair, so
  * that predict / objective use exactly those values
  * (probit/approximators.py:175 builds K + diag(1/precision)).  With

@@ -14,6 +14,7 @@ import torch
 
 MAX_DIM = 32
 MAX_CUT = 17
+NOALLOC_MAX_NP = 1024
 
 KERNEL_EQ = 0
 KERNEL_MATERN12 = 1
@@ -160,6 +161,10 @@ _SIGNATURES = {
     "pgp_grad_workspace_size": (_I64, [_I64, _I32]),
     "pgp_laplace_fit": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT), C.POINTER(LikelihoodT),
                                   C.POINTER(FitOptionsT), C.POINTER(FitResultT), _P, _I64, _P]),
+    "pgp_noalloc_scratch_bytes": (_I64, []),
+    "pgp_laplace_fit_noalloc": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT),
+                                          C.POINTER(LikelihoodT), C.POINTER(FitOptionsT),
+                                          C.POINTER(FitResultT), _P]),
     "pgp_laplace_set_state": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT),
                                         C.POINTER(LikelihoodT), _I32, _P, _P,
                                         C.POINTER(FitResultT), _P, _I64, _P]),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <list>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -835,6 +836,68 @@ int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
                 (long long)L.total);
   return laplace_fit_ws(problem, kernel, lik, opts, result, (char*)workspace,
                         (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Fit without a caller workspace: the workspace of pgp_laplace_fit_noalloc
+// is this file-scope device array, one copy per GPU the library runs on,
+// held until the process ends.  It is sized for the largest problem the
+// entry point takes (padded N <= PGP_NOALLOC_MAX_NP at any precision).  In
+// host code `scratch` names a host-side stand-in, not device memory: its
+// device address on the current GPU comes from hipGetSymbolAddress, once per
+// device, and only that address is ever handed to kernels or copies.
+// ---------------------------------------------------------------------------
+static_assert(PGP_NOALLOC_MAX_NP % 64 == 0, "the limit is a whole number of tiles");
+constexpr int64_t NOALLOC_BYTES = HEADER + 2 * (int64_t)PGP_NOALLOC_MAX_NP * PGP_NOALLOC_MAX_NP * 8 +
+                                  2 * (int64_t)PGP_NOALLOC_MAX_NP * 8 +
+                                  (int64_t)PGP_NOALLOC_MAX_NP * TILE * 8 +
+                                  (int64_t)NVEC * PGP_NOALLOC_MAX_NP * 8 +
+                                  (int64_t)PGP_NOALLOC_MAX_NP * (PGP_NOALLOC_MAX_NP / TILE) * 8 +
+                                  8 * 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on the current device (looked up on the
+// device's first call, with that device current).
+static int scratch_address(char** out) {
+  int dev = -1;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(-100, "device %d beyond the no-allocation table", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(-100, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = (char*)g_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" {
+
+int64_t pgp_noalloc_scratch_bytes(void) { return NOALLOC_BYTES; }
+
+int pgp_laplace_fit_noalloc(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
+                            const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
+                            pgp_fit_result_t* result, void* stream) {
+  int rc = check_fit_args(problem, kernel, lik, opts, result);
+  if (rc) return rc;
+  const WsLayout L = ws_layout(problem->N, opts->precision);
+  if (L.Np > PGP_NOALLOC_MAX_NP || L.total > NOALLOC_BYTES)
+    return fail(-1, "N = %lld exceeds the no-allocation limit (N <= %d)",
+                (long long)problem->N, PGP_NOALLOC_MAX_NP);
+  char* ws = nullptr;
+  rc = scratch_address(&ws);
+  if (rc) return rc;
+  return laplace_fit_ws(problem, kernel, lik, opts, result, ws, (hipStream_t)stream);
 }
 
 int pgp_vb_map(const pgp_problem_t* problem, const pgp_kernel_t* kernel,

@@ -170,6 +170,21 @@ CASES = [
         None, _gauss(L), FAKE, FAKE, 0, FAKE, 0, None), -1, "kernel is NULL"),
     ("vb_grad: likelihood NULL", lambda L, lib: lib.pgp_vb_grad(
         _kernel(L), None, FAKE, FAKE, 0, FAKE, 0, None), -2, "likelihood is NULL"),
+    ("noalloc: N beyond the limit", lambda L, lib: lib.pgp_laplace_fit_noalloc(
+        _problem(L, N=L.NOALLOC_MAX_NP + 1), _kernel(L), _gauss(L), _opts(L), L.FitResultT(),
+        None), -1, "no-allocation limit"),
+    ("noalloc: problem NULL", lambda L, lib: lib.pgp_laplace_fit_noalloc(
+        None, _kernel(L), _gauss(L), _opts(L), L.FitResultT(), None), -1, "problem is NULL"),
+    ("noalloc: options NULL", lambda L, lib: lib.pgp_laplace_fit_noalloc(
+        _problem(L), _kernel(L), _gauss(L), None, L.FitResultT(), None), -4,
+     "options are NULL"),
+    ("metrics: P NULL", lambda L, lib: lib.pgp_ordinal_metrics(None, 3, FAKE, 10, FAKE, None,
+                                                               FAKE, FAKE, None), -1,
+     "P is NULL"),
+    ("metrics: J", lambda L, lib: lib.pgp_ordinal_metrics(FAKE, 1, FAKE, 10, FAKE, None, FAKE,
+                                                          FAKE, None), -2, "J must be"),
+    ("metrics: workspace NULL", lambda L, lib: lib.pgp_ordinal_metrics(
+        FAKE, 3, FAKE, 10, FAKE, None, FAKE, None, None), -8, "workspace"),
     ("tune: precision", lambda L, lib: lib.pgp_tune_get(3, 0, L.TuneEntryT()), -1,
      "unknown precision"),
     ("tune: class", lambda L, lib: lib.pgp_tune_get(1, 2, L.TuneEntryT()), -2, "size class"),
