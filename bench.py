@@ -328,11 +328,13 @@ def main():
     lib.pgp_timer_enable(1 << _lib.TIMER_SYRK)
     timed = dev_reqs[W * R:(W + K) * R]
     results = {}
+    hs0 = eng.host_stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     results.update(serve_window(eng, timed))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    hs1 = eng.host_stats()
     lib.pgp_timer_enable(0)
     n_l, ms, fl, by = read_timer(lib, _lib, _lib.TIMER_SYRK)
     elapsed = t1 - t0
@@ -506,6 +508,14 @@ def main():
             "algo_bytes_per_launch": algo_bytes,
         },
         "kernel_classes": shares,
+        "host_loop": {  # the serving loop's host thread over the timed region
+            "lane_steps": hs1["lane_steps"] - hs0["lane_steps"],
+            "enqueue_share": round((hs1["enqueue_s"] - hs0["enqueue_s"]) / elapsed, 4),
+            "wait_for_gpu_share": round((hs1["wait_s"] - hs0["wait_s"]) / elapsed, 4),
+            "report_share": round((hs1["report_s"] - hs0["report_s"]) / elapsed, 4),
+            "enqueue_us_per_lane_step": round(1e6 * (hs1["enqueue_s"] - hs0["enqueue_s"]) /
+                                              max(1, hs1["lane_steps"] - hs0["lane_steps"]), 1),
+        },
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

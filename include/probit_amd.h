@@ -94,8 +94,21 @@ typedef struct pgp_fit_options {
   int32_t maxiter;   /* 100 = jaxopt's default */
   int32_t precision; /* PGP_PREC_* */
   int32_t refine;    /* FP64 iterative-refinement steps per solve */
-  int32_t flags;     /* reserved, 0 */
+  int32_t flags;     /* PGP_FIT_* bits, 0 by default */
 } pgp_fit_options_t;
+
+/* pgp_fit_options_t.flags: run every Newton iteration's launches directly,
+ * never from a captured HIP graph (the single-problem fit otherwise replays a
+ * graph of the iteration once its launch structure recurs in a later fit on
+ * the same workspace; the results are the same bits either way). */
+#define PGP_FIT_NO_GRAPH 1
+
+/* Counters of the Laplace fit's graph use since the library loaded: graphs
+ * captured, Newton iterations replayed from a graph and iterations launched
+ * directly.  No counterpart in the reference (whose iterations run under a
+ * JAX trace); for callers checking that repeated fits replay graphs. */
+int pgp_fit_graph_stats(int64_t* captured, int64_t* graph_iterations,
+                        int64_t* direct_iterations);
 
 /* Host-side result of a fit. */
 typedef struct pgp_fit_result {
@@ -453,6 +466,13 @@ int pgp_batcher_finish(pgp_batcher_t* batcher, int64_t* done_tags,
 int32_t pgp_batcher_launched(pgp_batcher_t* batcher);
 int32_t pgp_batcher_ready(pgp_batcher_t* batcher);
 int32_t pgp_batcher_in_flight(pgp_batcher_t* batcher);
+/* Host-side counters since creation, for measurement: seconds spent in
+ * pgp_batcher_launch (admission and enqueueing of lane steps), seconds
+ * pgp_batcher_finish waited for a lane's step to complete, seconds it spent
+ * reporting completed steps, and the number of lane steps launched.  Any
+ * output may be NULL.  No counterpart in the reference (measurement only). */
+int pgp_batcher_stats(pgp_batcher_t* batcher, double* host_enqueue_s, double* host_wait_s,
+                      double* host_report_s, int64_t* lane_steps);
 int pgp_batcher_destroy(pgp_batcher_t* batcher);
 
 #ifdef __cplusplus

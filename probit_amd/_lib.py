@@ -34,6 +34,8 @@ INFO_MAXITER = 4
 INFO_STALLED = 8
 INFO_ESCALATED = 16
 
+FIT_NO_GRAPH = 1
+
 TIMER_SYRK = 0
 TIMER_PANEL = 1
 TIMER_TRSV = 2
@@ -162,6 +164,7 @@ _SIGNATURES = {
     "pgp_laplace_fit": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT), C.POINTER(LikelihoodT),
                                   C.POINTER(FitOptionsT), C.POINTER(FitResultT), _P, _I64, _P]),
     "pgp_noalloc_scratch_bytes": (_I64, []),
+    "pgp_fit_graph_stats": (C.c_int, [C.POINTER(C.c_int64)] * 3),
     "pgp_laplace_fit_noalloc": (C.c_int, [C.POINTER(ProblemT), C.POINTER(KernelT),
                                           C.POINTER(LikelihoodT), C.POINTER(FitOptionsT),
                                           C.POINTER(FitResultT), _P]),
@@ -188,6 +191,8 @@ _SIGNATURES = {
     "pgp_batcher_launch": (C.c_int, [_P]),
     "pgp_batcher_finish": (C.c_int, [_P, C.POINTER(C.c_int64), _I32, C.POINTER(C.c_int32)]),
     "pgp_batcher_in_flight": (_I32, [_P]),
+    "pgp_batcher_stats": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "pgp_batcher_launched": (_I32, [_P]),
     "pgp_batcher_ready": (_I32, [_P]),
     "pgp_batcher_destroy": (C.c_int, [_P]),

@@ -7,13 +7,12 @@ columns; COLUMN_MAX decides which super-steps run one launch per block column
 and FUSE_MAX which update launches factor the next diagonal block inside the
 launch.  The library keeps one entry (G, COLUMN_MAX, FUSE_MAX) per precision
 and size class (padded N up to 1024, or above) in a mutex-guarded table
-(``pgp_tune_get`` / ``pgp_tune_set``).  ``tune()`` fills it from timed trials:
-for each class it serves a batch of synthetic requests shaped like the serving
-stream restricted to that class (N log-uniform over the class, D in 1..8,
-Gaussian or ordinal probit, EQ or Matern12, 256 test points) through the
-batcher to completion under each candidate, by coordinate search (G first,
-then COLUMN_MAX, then FUSE_MAX), keeps the fastest and persists the table as
-JSON per GPU architecture.  ``load()`` applies a persisted table; the package
+(``pgp_tune_get`` / ``pgp_tune_set``).  ``tune()`` fills it from timed trials
+on the workload it serves: a window of the serving stream (N log-uniform on
+[64, 4096], D in 1..8, Gaussian or ordinal probit, EQ or Matern12, 256 test
+points) through the asynchronous engine, under each candidate, by coordinate
+search per size class (G first, then COLUMN_MAX, then FUSE_MAX), keeps the
+fastest and persists the table as JSON per GPU architecture.  ``load()`` applies a persisted table; the package
 ships the one tuned on an MI355X (``tune_table.json``), which
 ``ServingEngine`` loads by default.
 """
@@ -26,7 +25,6 @@ import os
 import time
 from pathlib import Path
 
-import numpy as np
 import torch
 
 from . import _lib
@@ -104,51 +102,53 @@ def load(path=None, arch=None) -> bool:
     return True
 
 
-def _class_problems(size_class: int, n: int, seed: int):
-    """Synthetic requests of one size class, shaped like the serving stream."""
+def _stream_window(n: int, seed: int):
+    """Device copies of the first n requests of the serving stream."""
     from .data import serving_stream
-    lo, hi = (64, _lib.TUNE_CLASS_NP) if size_class == 0 else (_lib.TUNE_CLASS_NP + 1, 4096)
-    return [r.problem for r in serving_stream(seed=seed, n=n, Nmin=lo, Nmax=hi)]
+    out = []
+    for r in serving_stream(seed=seed, n=n):
+        p = r.problem
+        out.append((r.rid, torch.as_tensor(p.X, device="cuda"), torch.as_tensor(p.y, device="cuda"),
+                    p.kind, p.lengthscale, p.variance, p.lik, p.noise_std, p.cutpoints,
+                    torch.as_tensor(p.Xtest, device="cuda"), p.period))
+    return out
 
 
-def _serve_seconds(problems, dev, precision: str, reps: int) -> float:
-    """Best wall time of serving `problems` to completion through the batcher."""
-    from .serving import ServingEngine
+def _serve_seconds(window, precision: str, capacity: int, reps: int) -> float:
+    """Best wall time of serving the window through the asynchronous engine."""
+    from .serving import ServingEngine, serve_async
     best = math.inf
     for _ in range(reps):
-        eng = ServingEngine(precision=precision, capacity=len(problems))
+        eng = ServingEngine(precision=precision, capacity=capacity)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for tag, (p, X, y, Xt) in enumerate(zip(problems, *dev)):
-            if not eng.submit(tag, X, y, p.kind, p.lengthscale, p.variance, p.lik, p.noise_std,
-                              p.cutpoints, Xt, p.period):
-                raise RuntimeError("autotune: a trial request did not fit the device")
-        while eng.in_flight:
-            eng.step()
+        done = serve_async(eng, window)
         torch.cuda.synchronize()
         best = min(best, time.perf_counter() - t0)
         eng.close()
+        if len(done) != len(window):
+            raise RuntimeError("autotune: the trial window did not complete")
     return best
 
 
-def tune(precisions=("fp32",), classes=(0, 1), candidates=None, requests=(96, 16), reps=2,
-         seed=0, path=None, persist=True, log=None) -> dict:
+def tune(precisions=("fp32",), classes=(0, 1), candidates=None, window=1536, capacity=1024,
+         reps=2, seed=5, path=None, persist=True, log=None) -> dict:
     """Coordinate search of (superstep, column_max, fuse_max) per precision and
-    size class; sets the best entries in the library, persists them (``path``,
-    default the package table) and returns {precision: [entries]} with the
-    trial times."""
+    size class.  The objective is the serving workload itself: a window of the
+    serving stream (seed `seed`, `window` requests, mixed sizes in both
+    classes) served through the asynchronous engine with `capacity` slots;
+    each class's entry is searched field by field with the other class's
+    fixed.  Sets the best entries in the library, persists them (``path``,
+    default the package table) and returns {precision: [per-class result]}."""
     cand = dict(CANDIDATES, **(candidates or {}))
+    win = _stream_window(window, seed)
     out = {}
     for prec in precisions:
         out[prec] = []
+        _serve_seconds(win, prec, capacity, 1)  # warm-up: kernels loaded, allocator primed
+        t_best = _serve_seconds(win, prec, capacity, reps)
         for cls in classes:
-            probs = _class_problems(cls, requests[cls], seed + cls)
-            dev = ([torch.as_tensor(p.X, device="cuda") for p in probs],
-                   [torch.as_tensor(p.y, device="cuda") for p in probs],
-                   [torch.as_tensor(p.Xtest, device="cuda") for p in probs])
             best = get(prec, cls)
-            _serve_seconds(probs, dev, prec, 1)  # warm-up: kernels loaded, allocator primed
-            t_best = _serve_seconds(probs, dev, prec, reps)
             trials = [(dict(best), t_best)]
             for field in _FIELDS:
                 for v in cand[field]:
@@ -156,7 +156,7 @@ def tune(precisions=("fp32",), classes=(0, 1), candidates=None, requests=(96, 16
                         continue
                     e = dict(best, **{field: v})
                     set_entry(prec, cls, e)
-                    t = _serve_seconds(probs, dev, prec, reps)
+                    t = _serve_seconds(win, prec, capacity, reps)
                     trials.append((e, t))
                     if log:
                         log(f"autotune {prec} class {cls}: {e} {1e3 * t:.2f} ms")
@@ -166,6 +166,5 @@ def tune(precisions=("fp32",), classes=(0, 1), candidates=None, requests=(96, 16
             out[prec].append({"entry": best, "seconds": t_best,
                               "trials": [[e, round(t, 6)] for e, t in trials]})
     if persist:
-        tab = table()
-        save(path, tab=tab)
+        save(path, tab=table())
     return out

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <list>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -734,6 +735,113 @@ int64_t pgp_grad_workspace_size(int64_t N, int32_t D) {
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// HIP graphs of the single-problem Newton iteration.  An iteration's launches
+// depend only on the workspace, the padded size, the factor precision, whether
+// B is re-factored and the refinement count (the parameters live in the
+// device descriptor), so one captured graph serves every later iteration with
+// the same key, in this fit and in later fits on the same workspace
+// (optimisation loops, sweeps).  Replays are valid as they are: every
+// wavefront launch of an iteration has its own epoch, a member's progress
+// word is written by at least two of them per iteration (the two triangular
+// sweeps; a factor whose column launches wait on it has two or more of
+// those), so no replay finds its own epoch left from the previous one, and
+// the ticket counters reset themselves at each launch's end.  Capturing and
+// instantiating costs more than launching once, so a key is captured only
+// when it recurs in a later fit; a one-off fit runs its launches directly.
+// Off with PGP_FIT_NO_GRAPH in the options' flags (or PGP_GRAPH=0), and
+// whenever the launch timer is on.
+// ---------------------------------------------------------------------------
+struct GraphKey {
+  int dev;
+  const char* ws;
+  int Np, prec, refactor, refine;
+  pgp_tune_entry_t te;  // the schedule the factor's launches follow
+  bool operator==(const GraphKey& o) const {
+    return dev == o.dev && ws == o.ws && Np == o.Np && prec == o.prec &&
+           refactor == o.refactor && refine == o.refine && te.superstep == o.te.superstep &&
+           te.column_max == o.te.column_max && te.fuse_max == o.te.fuse_max;
+  }
+};
+struct GraphEntry {
+  GraphKey key;
+  int fit_seen;            // the last fit that used the key
+  int uses;                // fits that used the key
+  hipGraphExec_t exec;     // captured once the key recurs in a later fit
+  uint64_t stamp;
+};
+struct GraphCache {
+  std::mutex mu;
+  std::vector<GraphEntry> e;
+  hipStream_t capture[64] = {};
+  uint64_t clock = 0;
+  int fit_counter = 0;
+};
+static GraphCache& graph_cache() {
+  static GraphCache c;
+  return c;
+}
+constexpr size_t GRAPH_CACHE_MAX = 64;
+static std::atomic<int64_t> g_graphs_captured{0}, g_graph_launches{0}, g_direct_iters{0};
+
+
+static bool graphs_enabled(int flags) {
+  static const bool env = [] {
+    const char* e = getenv("PGP_GRAPH");
+    return !(e && e[0] == '0');
+  }();
+  if (!env || (flags & PGP_FIT_NO_GRAPH)) return false;
+  for (int c = 0; c < TC_COUNT; ++c)
+    if (timer_on(c)) return false;
+  return true;
+}
+
+// The graph of key k for fit `fit` (captured now if the key was used by an
+// earlier fit), or nullptr: run the launches directly.
+static hipGraphExec_t newton_graph(const GraphKey& k, int fit, const Batch& b, int prec,
+                                   int refine, double* gather) {
+  GraphCache& gc = graph_cache();
+  std::lock_guard<std::mutex> lock(gc.mu);
+  GraphEntry* ent = nullptr;
+  for (GraphEntry& g : gc.e)
+    if (g.key == k) ent = &g;
+  if (!ent) {
+    if (gc.e.size() >= GRAPH_CACHE_MAX) {  // evict the least recently used
+      size_t lru = 0;
+      for (size_t i = 1; i < gc.e.size(); ++i)
+        if (gc.e[i].stamp < gc.e[lru].stamp) lru = i;
+      if (gc.e[lru].exec) (void)hipGraphExecDestroy(gc.e[lru].exec);
+      gc.e.erase(gc.e.begin() + lru);
+    }
+    gc.e.push_back(GraphEntry{k, fit, 1, nullptr, ++gc.clock});
+    return nullptr;
+  }
+  ent->stamp = ++gc.clock;
+  if (ent->fit_seen != fit) {
+    ent->fit_seen = fit;
+    ++ent->uses;
+  }
+  if (ent->exec || ent->uses < 2) return ent->exec;
+  if (k.dev < 0 || k.dev >= 64) return nullptr;
+  if (!gc.capture[k.dev] &&
+      hipStreamCreateWithFlags(&gc.capture[k.dev], hipStreamNonBlocking) != hipSuccess)
+    return nullptr;
+  hipStream_t cs = gc.capture[k.dev];
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  newton_iter(b, k.refactor ? &b : nullptr, nullptr, k.refactor ? nullptr : &b, prec, refine,
+              gather, cs);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(cs, &g);
+  hipGraphExec_t exec = nullptr;
+  if (e == hipSuccess && g && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess) {
+    ent->exec = exec;
+    ++g_graphs_captured;
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();  // a failed capture leaves the launches to run directly
+  return ent->exec;
+}
+
 // Argument checks of the Laplace fit entry points (the workspace aside).
 static int check_fit_args(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
                           const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
@@ -767,6 +875,14 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
   rc = check_launch();
   if (rc) return rc;
   memset(result, 0, sizeof *result);
+  const bool use_graphs = graphs_enabled(opts->flags);
+  int dev = -1;
+  if (use_graphs) HIP_OK(hipGetDevice(&dev));
+  int fit_id = 0;
+  if (use_graphs) {
+    std::lock_guard<std::mutex> lock(graph_cache().mu);
+    fit_id = ++graph_cache().fit_counter;
+  }
   int it = 0;
   double step = INFINITY;
   const bool gauss = lik->kind == PGP_LIK_GAUSSIAN;
@@ -775,8 +891,19 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
   while (step > opts->tolerance && it < opts->maxiter) {
     const bool refactor = !gauss || it == 0 || force_factor;
     force_factor = false;
-    newton_iter(b, refactor ? &b : nullptr, nullptr, refactor ? nullptr : &b, cur, refine,
-                gather, s);
+    hipGraphExec_t gexec = nullptr;
+    if (use_graphs)
+      gexec = newton_graph(GraphKey{dev, ws, (int)p.Np, cur, refactor ? 1 : 0, refine,
+                                    tune_get(cur, tune_class_of((int)p.Np))},
+                           fit_id, b, cur, refine, gather);
+    if (gexec) {
+      HIP_OK(hipGraphLaunch(gexec, s));
+      ++g_graph_launches;
+    } else {
+      ++g_direct_iters;
+      newton_iter(b, refactor ? &b : nullptr, nullptr, refactor ? nullptr : &b, cur, refine,
+                  gather, s);
+    }
     rc = check_launch();
     if (rc) return rc;
     double h2;
@@ -822,6 +949,14 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
 }
 
 extern "C" {
+
+int pgp_fit_graph_stats(int64_t* captured, int64_t* graph_iterations, int64_t* direct_iterations) {
+  if (!captured || !graph_iterations || !direct_iterations) return fail(-1, "NULL argument");
+  *captured = g_graphs_captured.load();
+  *graph_iterations = g_graph_launches.load();
+  *direct_iterations = g_direct_iters.load();
+  return 0;
+}
 
 int pgp_laplace_fit(const pgp_problem_t* problem, const pgp_kernel_t* kernel,
                     const pgp_likelihood_t* lik, const pgp_fit_options_t* opts,
@@ -1395,6 +1530,10 @@ struct pgp_batcher {
   int cap;
   hipStream_t caller = nullptr;     // the stream given at creation
   hipEvent_t ev_caller = nullptr;   // its state at each launch / step
+  // host-side counters (pgp_batcher_stats): seconds spent enqueueing lane
+  // steps and reporting them, seconds finish() waited for the GPU, steps
+  double host_enqueue_s = 0.0, host_wait_s = 0.0, host_report_s = 0.0;
+  int64_t lane_steps = 0;
 
   int lane_np[pgp::NLANES - 1];  // size class bounds on Np (0: unused)
   pgp::Lane lane[pgp::NLANES];
@@ -1858,6 +1997,7 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
 
 int pgp_batcher_launch(pgp_batcher_t* bt) {
   if (!bt) return fail(-1, "batcher is NULL");
+  const auto t0 = std::chrono::steady_clock::now();
   admit(bt);
   if (bt->items.empty()) return 0;
   HIP_OK(hipEventRecord(bt->ev_caller, bt->caller));
@@ -1865,7 +2005,20 @@ int pgp_batcher_launch(pgp_batcher_t* bt) {
     if (bt->lane[li].launched) continue;
     const int rc = lane_launch(bt, li);
     if (rc) return rc;
+    if (bt->lane[li].launched) ++bt->lane_steps;
   }
+  bt->host_enqueue_s +=
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int pgp_batcher_stats(pgp_batcher_t* bt, double* host_enqueue_s, double* host_wait_s,
+                      double* host_report_s, int64_t* lane_steps) {
+  if (!bt) return fail(-1, "batcher is NULL");
+  if (host_enqueue_s) *host_enqueue_s = bt->host_enqueue_s;
+  if (host_wait_s) *host_wait_s = bt->host_wait_s;
+  if (host_report_s) *host_report_s = bt->host_report_s;
+  if (lane_steps) *lane_steps = bt->lane_steps;
   return 0;
 }
 
@@ -1876,6 +2029,7 @@ int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
   *n_done = 0;
   if (!pgp_batcher_launched(bt)) return 0;
   // wait until at least one launched lane has finished its step
+  const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
     bool any = false;
     for (int li = 0; li < NLANES; ++li) {
@@ -1884,7 +2038,14 @@ int pgp_batcher_finish(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done,
       const hipError_t e = hipEventQuery(L.ev);
       if (e == hipErrorNotReady) continue;
       if (e != hipSuccess) return fail(-100, "lane %d step failed: %s", li, hipGetErrorString(e));
+      if (!any) {
+        const auto t1 = std::chrono::steady_clock::now();
+        bt->host_wait_s += std::chrono::duration<double>(t1 - t0).count();
+      }
+      const auto t2 = std::chrono::steady_clock::now();
       lane_finish(bt, li, done_tags, max_done, n_done);
+      bt->host_report_s +=
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t2).count();
       any = true;
     }
     if (any) return 0;

@@ -405,12 +405,14 @@ void launch_pred_cov(const Batch& b, int prec, int maxMp, double* cov, int64_t l
 // the factor's rounding error; instead, with z_j = (L L^T)^-1 s_j from the
 // two FP32 solves,
 //   q_j = 2 s_j^T z_j - z_j^T B z_j = s_j^T B^-1 s_j - e^T B e,  e = z_j - B^-1 s_j,
-// whose error is quadratic in the error of z_j.  One workgroup per 64 x 64
-// tile (I, J) of K U, U = W^1/2 Z, as an FP64 MFMA product over the Np
-// columns of K (FP64), then per entry z (2 s - z - W^1/2 (K U)) with s
-// recomputed in FP64 from the inputs, summed over the tile's 64 rows in a
-// fixed order into qpart[I][j].  Z (the predict buffer after both solves)
-// holds z in the factor's storage type.
+// whose error is quadratic in the error of z_j.  With u = W^1/2 z,
+// z^T B z = z^T z + u^T K u, and K is symmetric: u^T K u = sum_I u_I^T A_I with
+// A_I = K_II u_I + 2 sum_{J < I} K_IJ u_J, so row tile I needs only the K
+// tiles on and left of the diagonal.  One workgroup per 64 x 64 tile (I, J)
+// of A, an FP64 MFMA product over those tiles of K (FP64), then per entry
+// z (2 s - z) - u A with s recomputed in FP64 from the inputs, summed over the
+// tile's 64 rows in a fixed order into qpart[I][j].  Z (the predict buffer
+// after both solves) holds z in the factor's storage type.
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void k_pred_refine(Prob* const* probs, Sched sc) {
   int bx;
@@ -431,10 +433,12 @@ __global__ __launch_bounds__(NTHREADS) void k_pred_refine(Prob* const* probs, Sc
   const int kl = threadIdx.x & 63, cg = threadIdx.x >> 6;
   doublex4 acc[2][2];
   zero_acc<double>(acc);
-  // ra[q] = K(64 ti + kl, k0 + cg + 4q); rb[q] = U(k0 + kl, 64 tj + cg + 4q)
+  // ra[q] = K(64 ti + kl, k0 + cg + 4q); rb[q] = c U(k0 + kl, 64 tj + cg + 4q),
+  // c = 2 left of the diagonal tile, 1 on it
   double ra[16], rb[16];
+  const int kend = ti * TILE;  // the diagonal tile's first column
   auto fetch = [&](int k0) {
-    const double w = Ws[k0 + kl];
+    const double w = (k0 < kend ? 2.0 : 1.0) * Ws[k0 + kl];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int c = cg + 4 * q;
@@ -443,7 +447,7 @@ __global__ __launch_bounds__(NTHREADS) void k_pred_refine(Prob* const* probs, Sc
     }
   };
   fetch(0);
-  for (int k0 = 0; k0 < P.Np; k0 += TILE) {
+  for (int k0 = 0; k0 <= kend; k0 += TILE) {
     __syncthreads();  // the previous chunk's products are done with LDS
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(NTHREADS) void k_pred_refine(Prob* const* probs, Sc
       sb[kl * PA + c] = rb[q];  // B(k = kl, n = c), k-major
     }
     __syncthreads();
-    if (k0 + TILE < P.Np) fetch(k0 + TILE);
+    if (k0 + TILE <= kend) fetch(k0 + TILE);
     mma64_kmajor<double>(sa, PA, sb, PA, acc);
   }
   __syncthreads();  // operands are dead: the input rows take their place
@@ -479,7 +483,7 @@ __global__ __launch_bounds__(NTHREADS) void k_pred_refine(Prob* const* probs, Sc
           const double z = (double)Z[(size_t)i + (size_t)j * np];
           const double sij = wi * kernel_value<double>(P.kern_kind, P.periodic, D, P.variance, ks,
                                                        &xs[row * D], &zs[col * D]);
-          part[bn] += z * (2.0 * sij - z - wi * acc[bm][bn][r]);
+          part[bn] += z * (2.0 * sij - z) - wi * z * acc[bm][bn][r];
         }
       }
   }
@@ -510,13 +514,15 @@ void launch_pred_refine(const Batch& b, int prec, int maxM, hipStream_t s) {
       [&](Prob* const* dp, const Sched& sc, int n) {
         double fl = 0.0, by = 0.0;
         if (on) {
-          // algorithmic work: the K U product, 2 Np^2 Mp flops; bytes: K read once
-          // per column tile, Z read twice
+          // algorithmic work: the quadratic forms u^T K u over the lower
+          // triangle of K, Np (Np + 64) Mp flops; bytes: the lower tiles of K
+          // read once per column tile, Z read twice
           const int off = (int)(dp - b.dprobs);
           for (int m = off; m < off + sc.count; ++m) {
             const double np = b.np(m), mp = (b.mcount(m, maxM) + TILE - 1) / TILE * TILE;
-            fl += 2.0 * np * np * mp;
-            by += np * np * 8.0 * (mp / TILE) + 2.0 * np * mp * (prec == PGP_PREC_FP64 ? 8 : 4);
+            fl += np * (np + TILE) * mp;
+            by += 0.5 * np * (np + TILE) * 8.0 * (mp / TILE) +
+                  2.0 * np * mp * (prec == PGP_PREC_FP64 ? 8 : 4);
           }
           timer_begin(TC_REFINE, s);
         }

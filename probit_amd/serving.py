@@ -226,6 +226,17 @@ class ServingEngine:
         """True if a lane's step is done (finish() would return at once)."""
         return self._launched and self.lib.pgp_batcher_ready(self._h) > 0
 
+    def host_stats(self) -> dict:
+        """Host-side seconds since creation: enqueueing lane steps (launch),
+        waiting for the GPU in finish(), reporting completed steps, and the
+        lane steps launched."""
+        e, w, r = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_int64()
+        _lib.check(self.lib.pgp_batcher_stats(self._h, C.byref(e), C.byref(w), C.byref(r),
+                                              C.byref(n)), "pgp_batcher_stats")
+        return {"enqueue_s": e.value, "wait_s": w.value, "report_s": r.value,
+                "lane_steps": n.value}
+
     @property
     def in_flight(self) -> int:
         return len(self._slots)
@@ -240,6 +251,30 @@ class ServingEngine:
             self.close()
         except Exception:
             pass
+
+
+def serve_async(engine: ServingEngine, requests, chunk=128):
+    """Serve `requests` -- (tag, X, y, kind, lengthscale, variance, lik,
+    noise_std, cutpoints, Xtest, period) tuples with device tensors -- to
+    completion with asynchronous steps: while the GPU runs a step the host
+    offers the next requests in order (a refusal, or a lane's step being
+    done, ends a round of offers).  Returns {tag: Completed}."""
+    pending = list(requests)
+    done = {}
+    while pending or engine.in_flight:
+        launched = engine.launch()
+        n = 0
+        while n < len(pending) and n < chunk:
+            if n % 8 == 7 and engine.ready():
+                break
+            if not engine.submit(*pending[n]):
+                break
+            n += 1
+        pending = pending[n:]
+        if launched:
+            for c in engine.finish():
+                done[c.tag] = c
+    return done
 
 
 def serve_stream(engine: ServingEngine, requests, realtime=True, on_done=None):

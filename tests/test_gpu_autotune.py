@@ -107,7 +107,7 @@ def test_tuning_run_persists_a_table_that_loads_back(restore_table, tmp_path):
     path = tmp_path / "tune.json"
     res = at.tune(precisions=("fp32",), classes=(0,),
                   candidates={"superstep": (4, 8), "column_max": (0, 512), "fuse_max": (512,)},
-                  requests=(12, 2), reps=1, path=path)
+                  window=24, capacity=16, reps=1, path=path)
     best = res["fp32"][0]["entry"]
     assert best["superstep"] in (4, 8) and best["column_max"] in (0, 512)
     assert len(res["fp32"][0]["trials"]) >= 3
