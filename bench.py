@@ -83,6 +83,10 @@ def parse():
                         "one, median of REPEATS runs) and print it")
     p.add_argument("--seed", type=int, default=4)
     p.add_argument("--stream", type=int, default=4096)
+    p.add_argument("--kernel-window", type=int, default=0,
+                   help="only serve this many requests of the timed set with synchronous steps "
+                        "(the same launches on every run: the command the per-kernel profiles "
+                        "and counter runs use) and exit")
     p.add_argument("--min-n", type=int, default=0,
                    help="serve only the stream's requests with at least this many points "
                         "(0: the whole stream)")
@@ -134,6 +138,24 @@ def serve_window(engine, window, t_arrive=None, chunk=128):
             start = t_sub[c.tag] if t_arrive is None else t_arrive[c.tag]
             res[c.tag] = (t_done - start, c)
     return res
+
+
+def kernel_window(engine, window):
+    """Serve `window` with synchronous steps, offering requests in order as
+    slots free up: the launch sequence depends only on the requests, not on
+    timing.  Returns {rid: Completed}."""
+    pending, done = list(window), {}
+    while pending or engine.in_flight:
+        while pending:
+            r = pending[0]
+            p = r["p"]
+            if not engine.submit(r["rid"], r["X"], r["y"], p.kind, p.lengthscale, p.variance,
+                                 p.lik, p.noise_std, p.cutpoints, r["Xt"], p.period):
+                break
+            pending.pop(0)
+        for c in engine.step():
+            done[c.tag] = c
+    return done
 
 
 def plain_fp64(sample, torch, _lib):
@@ -241,10 +263,12 @@ def cpu_study(problems, strata, repeats):
     for variant in ("lu", "bform"):
         for threads in (None, 1):
             rates = []
-            for _ in range(repeats):
+            key = f"{variant}_{'all' if threads is None else 1}_cores"
+            for rep in range(repeats):
                 sample, t, cores = cpu_baseline(problems, strata, variant, threads)
                 rates.append(len(t) / sum(t))
-            key = f"{variant}_{'all' if threads is None else 1}_cores"
+                print(f"cpu_study {key} run {rep}: {rates[-1]:.4f} posteriors/s", file=sys.stderr,
+                      flush=True)
             out["runs"][key] = {"cores": cores, "posteriors_per_s_median": float(np.median(rates)),
                                 "runs": [round(r, 4) for r in rates]}
     out["sample_mean_N"] = float(np.mean([p.N for p in _strata(problems, strata)]))
@@ -320,6 +344,17 @@ def main():
 
     eng = ServingEngine(precision=args.precision, capacity=args.capacity,
                         refine=None if args.refine < 0 else args.refine)
+    if args.kernel_window:
+        win = dev_reqs[W * R:W * R + args.kernel_window]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = kernel_window(eng, win)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel_window": {"requests": len(done), "seconds":
+                                            round(time.perf_counter() - t0, 4),
+                                            "precision": args.precision,
+                                            "timed_set": _mix([r["p"] for r in win])}}))
+        return
     # warmup windows
     for s in range(W):
         serve_window(eng, dev_reqs[s * R:(s + 1) * R])

@@ -171,8 +171,10 @@ int pgp_timer_read(int32_t cls, int64_t* launches, double* total_ms,
  * column_max: super-steps whose first column has at most this many
  * workgroups run one launch per block column (0: never); fuse_max: update
  * launches of at most this many tiles factor the next diagonal block inside
- * the launch (0: never).  column_max and fuse_max change only the launches,
- * never the results.  The table is process-wide, read at every
+ * the launch (0: never); column_merge (0 or 1): such column-form
+ * super-steps run as one launch with in-launch tile hand-offs (1) or one
+ * launch per block column (0).  column_max, fuse_max and column_merge change
+ * only the launches, never the results.  The table is process-wide, read at every
  * factorisation and guarded by a mutex; probit_amd.autotune fills it from
  * timed trials and persists it. */
 #define PGP_TUNE_CLASSES 2
@@ -181,7 +183,7 @@ typedef struct pgp_tune_entry {
   int32_t superstep;
   int32_t column_max;
   int32_t fuse_max;
-  int32_t reserved;
+  int32_t column_merge;
 } pgp_tune_entry_t;
 int pgp_tune_get(int32_t precision, int32_t size_class, pgp_tune_entry_t* out);
 int pgp_tune_set(int32_t precision, int32_t size_class, const pgp_tune_entry_t* entry);

@@ -130,7 +130,7 @@ class TuneEntryT(C.Structure):
         ("superstep", C.c_int32),
         ("column_max", C.c_int32),
         ("fuse_max", C.c_int32),
-        ("reserved", C.c_int32),
+        ("column_merge", C.c_int32),
     ]
 
 

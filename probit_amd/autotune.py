@@ -3,15 +3,17 @@
 The factorisation inside every Newton step (the reference's ``B.cholesky``,
 probit/implicit/Laplace.py:24, and the LU solve of the Newton system,
 probit/implicit/solvers.py:24) runs in super-steps of G 64-column block
-columns; COLUMN_MAX decides which super-steps run one launch per block column
-and FUSE_MAX which update launches factor the next diagonal block inside the
-launch.  The library keeps one entry (G, COLUMN_MAX, FUSE_MAX) per precision
+columns; COLUMN_MAX decides which super-steps run in the column kernel,
+COLUMN_MERGE whether such a super-step is one launch or one launch per block
+column, and FUSE_MAX which update launches factor the next diagonal block
+inside the launch.  The library keeps one entry (G, COLUMN_MAX, FUSE_MAX,
+COLUMN_MERGE) per precision
 and size class (padded N up to 1024, or above) in a mutex-guarded table
 (``pgp_tune_get`` / ``pgp_tune_set``).  ``tune()`` fills it from timed trials
 on the workload it serves: a window of the serving stream (N log-uniform on
 [64, 4096], D in 1..8, Gaussian or ordinal probit, EQ or Matern12, 256 test
 points) through the asynchronous engine, under each candidate, by coordinate
-search per size class (G first, then COLUMN_MAX, then FUSE_MAX), keeps the
+search per size class (G, then COLUMN_MAX, FUSE_MAX and COLUMN_MERGE), keeps the
 fastest and persists the table as JSON per GPU architecture.  ``load()`` applies a persisted table; the package
 ships the one tuned on an MI355X (``tune_table.json``), which
 ``ServingEngine`` loads by default.
@@ -33,9 +35,11 @@ __all__ = ["CANDIDATES", "PACKAGE_TABLE", "device_arch", "get", "set_entry", "ta
            "load", "tune"]
 
 CANDIDATES = {"superstep": (2, 3, 4, 6, 8, 12, 16), "column_max": (0, 256, 512, 1024, 4096),
-              "fuse_max": (0, 512)}
+              "fuse_max": (0, 512), "column_merge": (0, 1)}
 PACKAGE_TABLE = Path(__file__).resolve().parent / "tune_table.json"
-_FIELDS = ("superstep", "column_max", "fuse_max")
+_FIELDS = ("superstep", "column_max", "fuse_max", "column_merge")
+# a persisted entry without a field takes this value
+_DEFAULTS = {"column_merge": 1}
 _PRECISIONS = ("fp64", "fp32", "bf16")
 
 
@@ -56,7 +60,7 @@ def set_entry(precision: str, size_class: int, entry: dict) -> None:
     lib = _lib.load()
     e = _lib.TuneEntryT()
     for f in _FIELDS:
-        setattr(e, f, int(entry[f]))
+        setattr(e, f, int(entry[f] if f in entry else _DEFAULTS[f]))
     _lib.check(lib.pgp_tune_set(_lib.PRECISIONS[precision], size_class, e), "pgp_tune_set")
 
 
@@ -133,7 +137,7 @@ def _serve_seconds(window, precision: str, capacity: int, reps: int) -> float:
 
 def tune(precisions=("fp32",), classes=(0, 1), candidates=None, window=1536, capacity=1024,
          reps=2, seed=5, path=None, persist=True, log=None) -> dict:
-    """Coordinate search of (superstep, column_max, fuse_max) per precision and
+    """Coordinate search of (superstep, column_max, fuse_max, column_merge) per precision and
     size class.  The objective is the serving workload itself: a window of the
     serving stream (seed `seed`, `window` requests, mixed sizes in both
     classes) served through the asynchronous engine with `capacity` slots;

@@ -79,7 +79,7 @@ constexpr int64_t OFF_PROGRESS_CT = 2560;  // uint64[4] progress words of the wa
 constexpr int64_t HEADER = 4096;
 
 struct WsLayout {
-  int64_t Np, K, L, logdiag, diagval, dinv, vec, symv, total;
+  int64_t Np, K, L, logdiag, diagval, dinv, vec, symv, csync, total;
 };
 
 static WsLayout ws_layout(int64_t N, int prec) {
@@ -100,6 +100,8 @@ static WsLayout ws_layout(int64_t N, int prec) {
   off += align256((int64_t)NVEC * w.Np * 8);
   w.symv = off;
   off += align256(w.Np * (w.Np / TILE) * 8);
+  w.csync = off;
+  off += align256(csync_words(w.Np) * 4);
   w.total = off;
   return w;
 }
@@ -180,6 +182,8 @@ __global__ void k_write_prob(Prob* dst, Prob val, int reset) {
     // progress word and ticket counter of the header
     if (threadIdx.x < 2) reinterpret_cast<uint64_t*>((char*)dst + OFF_PROGRESS)[threadIdx.x] = 0;
     if (threadIdx.x < 4) reinterpret_cast<uint64_t*>((char*)dst + OFF_PROGRESS_CT)[threadIdx.x] = 0;
+    if (val.csync)
+      for (int64_t w = threadIdx.x; w < csync_words(val.Np); w += blockDim.x) val.csync[w] = 0;
   }
 }
 
@@ -210,6 +214,8 @@ __global__ void k_scatter_probs(Prob* const* dst, const Prob* src, int n, int re
     if (threadIdx.x < NRED) v.red[threadIdx.x] = 0.0;
     if (threadIdx.x == 0) v.progress[0] = 0;
     if (threadIdx.x < 4) v.progress_ct[threadIdx.x] = 0;
+    if (v.csync)
+      for (int64_t w = threadIdx.x; w < csync_words(v.Np); w += blockDim.x) v.csync[w] = 0;
   }
 }
 
@@ -311,6 +317,7 @@ static Prob make_prob(const pgp_problem_t* pr, const pgp_kernel_t* k, const pgp_
   p.dinv = (double*)(ws + L.dinv);
   p.vec = (double*)(ws + L.vec);
   p.symv = (double*)(ws + L.symv);
+  p.csync = (uint32_t*)(ws + L.csync);
   p.red = (double*)(ws + OFF_RED);
   p.info = (int32_t*)(ws + OFF_INFO);
   p.progress = (uint64_t*)(ws + OFF_PROGRESS);
@@ -597,9 +604,9 @@ int pgp_tune_set(int32_t precision, int32_t size_class, const pgp_tune_entry_t* 
     return fail(-3, "superstep must be in [1, 16] (got %d)", entry->superstep);
   if (entry->column_max < 0 || entry->fuse_max < 0)
     return fail(-3, "column_max / fuse_max must be >= 0");
-  pgp_tune_entry_t v = *entry;
-  v.reserved = 0;
-  tune_set(precision, size_class, v);
+  if (entry->column_merge != 0 && entry->column_merge != 1)
+    return fail(-3, "column_merge must be 0 or 1 (got %d)", entry->column_merge);
+  tune_set(precision, size_class, *entry);
   return 0;
 }
 
@@ -990,7 +997,7 @@ constexpr int64_t NOALLOC_BYTES = HEADER + 2 * (int64_t)PGP_NOALLOC_MAX_NP * PGP
                                   (int64_t)PGP_NOALLOC_MAX_NP * TILE * 8 +
                                   (int64_t)NVEC * PGP_NOALLOC_MAX_NP * 8 +
                                   (int64_t)PGP_NOALLOC_MAX_NP * (PGP_NOALLOC_MAX_NP / TILE) * 8 +
-                                  8 * 256;
+                                  csync_words(PGP_NOALLOC_MAX_NP) * 4 + 9 * 256;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
 constexpr int NOALLOC_MAX_DEVICES = 64;

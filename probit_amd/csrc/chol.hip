@@ -554,9 +554,17 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
 // computed as the trailing updates are (T / BF16 operands on MFMA, result
 // rounded to T), then held in LDS as FP64: out[r][c] (kmajor = false) or
 // out[c][r] (kmajor = true).  `arena` holds the operands first.
-template <typename T, int BF16, typename OT>
+// COH: the operand columns were written in this launch by other workgroups
+// (merged column kernel): they are read with coherent loads, and `hook(c)`
+// (called by every thread before column c's operands are fetched) waits
+// for them.  The arithmetic is the same either way.
+struct NoHook {
+  __device__ void operator()(int) const {}
+};
+template <typename T, int BF16, typename OT, bool COH = false, typename Hook = NoHook>
 __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int kc0, int kc1, int64_t ri,
-                                   int64_t rj, char* arena, OT* out, int opitch, bool kmajor) {
+                                   int64_t rj, char* arena, OT* out, int opitch, bool kmajor,
+                                   Hook hook = Hook()) {
   auto A = gbl(A_);
   constexpr bool MM = MmajorF32<T, BF16>::value;
   constexpr int PITCH = BF16 ? 72 : MM ? 68 : KP<T>::value;
@@ -570,12 +578,18 @@ __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int 
   if (kc1 > kc0) {
     T ra[16], rb[16];
     auto fetch = [&](int c) {
+      hook(c);
       const int64_t c0 = (int64_t)c * TILE;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int k = operand_k<MM>(kb, q);
-        ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
-        rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
+        if constexpr (COH) {
+          ra[q] = ld_coh(&A[(size_t)(ri + m) + (size_t)(c0 + k) * ld]);
+          rb[q] = ld_coh(&A[(size_t)(rj + m) + (size_t)(c0 + k) * ld]);
+        } else {
+          ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
+          rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
+        }
       }
     };
     fetch(kc0);
@@ -695,6 +709,166 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
         L[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(c0 + acc_col(bn)) * P.Np] = acc[bm][bn][r];
 }
 
+// ---- one launch per super-step (merged column kernel) ------------------------
+// The columns [s0, s1) of a super-step in one launch: every tile (i, j) of
+// those columns (i >= j) is one workgroup, in ticket order column by column
+// (column j: the diagonal workgroups of the members with more than j block
+// columns, then their row workgroups).  A workgroup applies the left-looking
+// update by the columns s0..j-1 as k_column does, fetching those columns'
+// tiles -- written in this launch -- with coherent loads once their flags
+// show them final; a diagonal workgroup then factors and inverts block j, a
+// row workgroup waits for that and forms L_ij = C_ij DINV_j^T.  Each tile's
+// flag is published after its stores complete.  The chain from one column to
+// the next is tile (j+1, j) -> block j+1, so a column's factor overlaps the
+// previous column's remaining rows, and the G - 1 launch gaps of the
+// per-column form go away.  Same operations in the same order on every tile
+// as k_column: the factor is bitwise the same.
+//
+// Member sync words (Prob::csync, zeroed when the descriptor is written with
+// reset): flag (i, c) at csync[(c - s0) nb + i] holds the launch's epoch
+// once tile (i, c) is final; csync[16 nb] counts the member's finished
+// workgroups.  The member's last workgroup resets the count and clears the
+// flags, so no flag outlives its launch (a replayed graph of the launch,
+// whose epoch is the captured one, starts from cleared flags).
+constexpr int MAXG = 16;
+struct StepSched {
+  int32_t count;             // members (sorted by size, descending)
+  int32_t s0, s1;            // columns [s0, s1)
+  int32_t colbase[MAXG + 1];  // first ticket of column s0 + c
+  int32_t nd[MAXG];           // members with more than s0 + c block columns
+  int32_t pre[MAXB + 1];      // pre[m]: block columns of the members before m
+};
+
+__device__ __forceinline__ uint32_t* tile_flag(const Prob& P, int nb, int s0, int i, int c) {
+  return P.csync + (size_t)(c - s0) * nb + i;
+}
+
+// Wave 0 waits (bounded, one flag per lane) until the flags of tiles (i, c)
+// and (j, c), c in [ca, cb), equal `epoch`; then the workgroup proceeds.
+// All threads call it.
+__device__ __forceinline__ void wait_tiles(const Prob& P, int nb, int s0, int i, int j, int ca,
+                                           int cb, uint32_t epoch) {
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
+    const int n = 2 * (cb - ca);
+    const uint32_t* f = l < n ? tile_flag(P, nb, s0, (l & 1) ? j : i, ca + (l >> 1)) : nullptr;
+    for (int spins = 0;; ++spins) {
+      const bool ok =
+          !f || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+      if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
+      if (spins >= SPIN_LIMIT) {
+        if (l == 0) atomicOr(P.info, PGP_INFO_STALLED);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  __syncthreads();
+}
+
+template <typename T, int BF16>
+__global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, StepSched ss,
+                                                        uint32_t* ticket, uint32_t epoch) {
+  __shared__ __attribute__((aligned(16))) double arena[2][TILE][TILE + 1];
+  __shared__ int s_val;
+  const int t = take_ticket(ticket, &s_val);
+  if ((unsigned)t >= gridDim.x) {  // counter not zero at launch: nothing to take
+    flag_members_stalled(probs, ss.count);
+    return;
+  }
+  // ticket -> (column j, member mi, row block i)
+  int jj = 0;
+  while (t >= ss.colbase[jj + 1]) ++jj;
+  const int s0 = ss.s0, j = s0 + jj;
+  int q = t - ss.colbase[jj];
+  int mi, i;
+  if (q < ss.nd[jj]) {
+    mi = q;
+    i = j;
+  } else {
+    // row workgroups of column j: member m has nb_m - j - 1, so the members
+    // before m hold pre[m] - (j + 1) m of them
+    q -= ss.nd[jj];
+    int lo = 0, hi = ss.nd[jj];
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ss.pre[mid] - (j + 1) * mid <= q)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    mi = lo;
+    i = j + 1 + (q - (ss.pre[lo] - (j + 1) * lo));
+  }
+  const Prob& P = *probs[mi];
+  const int nb = P.Np / TILE;
+  const int64_t c0 = (int64_t)j * TILE, ri = (int64_t)i * TILE;
+  // the columns before the last are normally final long before: one wait
+  // for all of them, then column j - 1's tiles right before they are fetched
+  if (j - 1 > s0) wait_tiles(P, nb, s0, i, j, s0, j - 1, epoch);
+  auto last_col = [&](int c) {
+    if (c == j - 1) wait_tiles(P, nb, s0, i, j, c, c + 1, epoch);
+  };
+  if (i == j) {
+    update_tile_to_lds<T, BF16, double, true>((const T*)P.L, P.Np, s0, j, c0, c0, (char*)arena,
+                                              &arena[0][0][0], TILE + 1, false, last_col);
+    FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
+                P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
+                P.info};
+    factor_block_blocked<T>(arena[0], arena[1], c0, o);
+  } else {
+    constexpr int PP = KP<T>::value;
+    T* ck = (T*)&arena[0][0][0];
+    T* bk = ck + TILE * PP;
+    update_tile_to_lds<T, BF16, T, true>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena, ck, PP,
+                                         true, last_col);
+    wait_tiles(P, nb, s0, j, j, j, j + 1, epoch);  // block j factored, DINV_j published
+    const double* Dj = P.dinv + (size_t)j * TILE * TILE;
+    {
+      const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
+      for (int qq = 0; qq < 16; ++qq) {
+        const int pp = pb + 4 * qq;
+        bk[pp * PP + n] = (T)ld_coherent(&Dj[n + pp * TILE]);
+      }
+    }
+    __syncthreads();
+    typename Acc<T>::type acc[2][2];
+    zero_acc<T>(acc);
+    mma64_kmajor<T>(ck, PP, bk, PP, acc);
+    T* L = (T*)P.L;
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          st_coh(&L[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(c0 + acc_col(bn)) * P.Np],
+                 (T)acc[bm][bn][r]);
+  }
+  // every wave's stores complete, then publish the tile
+  publish_wait();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(tile_flag(P, nb, s0, i, j), epoch, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    // the member's last workgroup in this launch clears its sync words
+    const int e = min(ss.s1, nb);
+    const uint32_t total = (uint32_t)((e - s0) * nb - (s0 + e - 1) * (e - s0) / 2);
+    uint32_t* done = P.csync + (size_t)MAXG * nb;
+    s_val = (atomicAdd(done, 1u) + 1u == total) ? 1 : 0;
+  }
+  __syncthreads();
+  if (s_val) {
+    const int e = min(ss.s1, nb);
+    for (int w = threadIdx.x; w < (e - s0) * nb; w += NTHREADS)
+      __hip_atomic_store(P.csync + w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0)
+      __hip_atomic_store(P.csync + (size_t)MAXG * nb, 0u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Diagonal block k (k >= 1) after the trailing update of step k - 1 (separate
 // mode): one workgroup per member.
 template <typename T>
@@ -780,16 +954,19 @@ __global__ __launch_bounds__(NTHREADS) void k_dinv_raw(const T* A, int64_t ld, d
 // ---------------------------------------------------------------------------
 // Schedule table (the autotuner's): per precision and size class, the
 // super-step width G (rank-64G outer updates), COLUMN_MAX (super-steps whose
-// first column has at most this many workgroups run one k_column launch per
-// column; 0 disables) and FUSE_MAX (update launches of at most this many
-// tiles factor the next diagonal block inside the launch; 0 disables).
+// first column has at most this many workgroups run in the column kernel;
+// 0 disables), FUSE_MAX (update launches of at most this many tiles factor
+// the next diagonal block inside the launch; 0 disables) and COLUMN_MERGE
+// (1: such a super-step is one k_superstep launch, 0: one k_column launch
+// per column).
 // Size class 0 holds the members with padded N <= TUNE_CLASS_NP, class 1 the
 // larger ones; a batch is factored class by class, so a member's schedule --
 // and with G its rounding -- depends only on its own size (a result is the
 // same alone and in any mix, in the batcher or the single-problem path).
-// COLUMN_MAX and FUSE_MAX change only the launches, never the arithmetic.
-// Initial values: PGP_SUPERSTEP / PGP_COLUMN_MAX / PGP_FUSE_MAX from the
-// environment, else 8 / 512 / 512.  Read at every factorisation, written
+// COLUMN_MAX, FUSE_MAX and COLUMN_MERGE change only the launches, never the
+// arithmetic.
+// Initial values: PGP_SUPERSTEP / PGP_COLUMN_MAX / PGP_FUSE_MAX /
+// PGP_COLUMN_MERGE from the environment, else 8 / 512 / 512 / 1.  Read at every factorisation, written
 // rarely (pgp_tune_set): a mutex guards it.
 // ---------------------------------------------------------------------------
 constexpr int TUNE_CLASSES = 2;
@@ -806,8 +983,9 @@ struct TuneTable {
   TuneTable() {
     const int G = std::min(16, std::max(1, env_int("PGP_SUPERSTEP", 8)));
     const int cm = env_int("PGP_COLUMN_MAX", 512), fm = env_int("PGP_FUSE_MAX", 512);
+    const int mg = env_int("PGP_COLUMN_MERGE", 1) ? 1 : 0;
     for (int p = 0; p < 3; ++p)
-      for (int c = 0; c < TUNE_CLASSES; ++c) e[p][c] = pgp_tune_entry_t{G, cm, fm, 0};
+      for (int c = 0; c < TUNE_CLASSES; ++c) e[p][c] = pgp_tune_entry_t{G, cm, fm, mg};
   }
 };
 static TuneTable& tune_table() {
@@ -860,6 +1038,43 @@ static void column_launch(const Batch& b, int j, int s0, bool tpa, hipStream_t s
     if (tpa) timer_begin(TC_PANEL, s);
     hipLaunchKernelGGL((k_column<T, BF16>), dim3(nd + tot), dim3(NTHREADS), 0, s, b.dprobs + off,
                        sc, nd, b.ticket, ep, j, s0);
+    if (tpa) timer_end(TC_PANEL, s, fl, 0.0);
+  }
+}
+
+template <typename T, int BF16>
+static void superstep_launch(const Batch& b, int s0, int s1, bool tpa, hipStream_t s) {
+  constexpr double F3 = TILE * TILE * TILE / 3.0;
+  constexpr double G64 = 2.0 * TILE * TILE * TILE;
+  for (int off = 0; off < b.count; off += MAXB) {
+    const int n = std::min(MAXB, b.count - off);
+    StepSched ss;
+    ss.count = n;
+    ss.s0 = s0;
+    ss.s1 = s1;
+    ss.pre[0] = 0;
+    for (int m = 0; m < n; ++m) ss.pre[m + 1] = ss.pre[m] + b.np(off + m) / TILE;
+    ss.colbase[0] = 0;
+    double fl = 0.0;
+    for (int j = s0; j < s1; ++j) {
+      int nd = 0, rows = 0;
+      for (int m = 0; m < n; ++m) {
+        const int nbm = b.np(off + m) / TILE;
+        if (nbm > j) {
+          ++nd;
+          rows += nbm - j - 1;
+          fl += F3 + (double)(nbm - j) * G64 * (j - s0) + (double)(nbm - j - 1) * G64;
+        }
+      }
+      ss.nd[j - s0] = nd;
+      ss.colbase[j - s0 + 1] = ss.colbase[j - s0] + nd + rows;
+    }
+    const int tot = ss.colbase[s1 - s0];
+    if (tot == 0) continue;
+    const uint32_t ep = next_epoch();
+    if (tpa) timer_begin(TC_PANEL, s);
+    hipLaunchKernelGGL((k_superstep<T, BF16>), dim3(tot), dim3(NTHREADS), 0, s, b.dprobs + off, ss,
+                       b.ticket, ep);
     if (tpa) timer_end(TC_PANEL, s, fl, 0.0);
   }
 }
@@ -943,7 +1158,11 @@ static void potrf_steps(const Batch& b, hipStream_t s, const pgp_tune_entry_t& t
     const int s0 = k / G * G, s_end = s0 + G;
     const bool outer = (k + 1 == s_end);
     if (col_style(s0)) {
-      column_launch<T, BF16>(b, k, s0, tpa, s);
+      if (te.column_merge) {
+        if (k == s0) superstep_launch<T, BF16>(b, s0, std::min(s_end, nb), tpa, s);
+      } else {
+        column_launch<T, BF16>(b, k, s0, tpa, s);
+      }
       if (outer && s_end < nb) outer_launch(s0, s_end, false);
       continue;
     }

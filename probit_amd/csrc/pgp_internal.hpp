@@ -100,7 +100,12 @@ struct Prob {
   // predict with the FP64 variance correction: (Np/64) x Mp partial sums
   // (NULL: the variance from the column norms of L^-1 S)
   double* qpart;
+  // merged column kernel of the Cholesky: 16 (Np/64) tile flags and a
+  // workgroup count, zero between launches
+  uint32_t* csync;
 };
+// words of Prob::csync for padded size Np
+__host__ __device__ constexpr inline int64_t csync_words(int64_t Np) { return 16 * (Np / 64) + 16; }
 
 __host__ __device__ inline double* vslot(const Prob& p, int v) {
   return p.vec + (size_t)v * (size_t)p.Np;
@@ -191,6 +196,16 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
 }
 __device__ __forceinline__ void st_coherent(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The same for factor values in the storage type (float or double), through
+// the global address space.
+template <typename T>
+__device__ __forceinline__ T ld_coh(const __attribute__((address_space(1))) T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_coh(T* p, T v) {
+  __hip_atomic_store(gbl(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // This wave's outstanding global stores (and loads) complete; also a
 // compiler barrier for memory.

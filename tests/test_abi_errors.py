@@ -326,10 +326,11 @@ def test_tune_table_round_trips_on_the_host():
     old = L.TuneEntryT()
     assert lib.pgp_tune_get(2, 1, old) == 0
     assert 1 <= old.superstep <= 16
-    assert lib.pgp_tune_set(2, 1, L.TuneEntryT(5, 100, 0, 77)) == 0
+    assert lib.pgp_tune_set(2, 1, L.TuneEntryT(5, 100, 0, 0)) == 0
     got = L.TuneEntryT()
     assert lib.pgp_tune_get(2, 1, got) == 0
-    assert (got.superstep, got.column_max, got.fuse_max, got.reserved) == (5, 100, 0, 0)
+    assert (got.superstep, got.column_max, got.fuse_max, got.column_merge) == (5, 100, 0, 0)
+    assert lib.pgp_tune_set(2, 1, L.TuneEntryT(5, 100, 0, 2)) == -3  # column_merge is 0 or 1
     other = L.TuneEntryT()
     assert lib.pgp_tune_get(2, 0, other) == 0 and other.superstep == old.superstep
     assert lib.pgp_tune_set(2, 1, old) == 0
