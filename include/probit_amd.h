@@ -171,9 +171,10 @@ int pgp_timer_read(int32_t cls, int64_t* launches, double* total_ms,
  * column_max: super-steps whose first column has at most this many
  * workgroups run one launch per block column (0: never); fuse_max: update
  * launches of at most this many tiles factor the next diagonal block inside
- * the launch (0: never); column_merge (0 or 1): such column-form
- * super-steps run as one launch with in-launch tile hand-offs (1) or one
- * launch per block column (0).  column_max, fuse_max and column_merge change
+ * the launch (0: never); column_merge: such column-form super-steps run
+ * one launch per block column (0), the super-step's diagonal block of tiles
+ * in one launch with in-launch tile hand-offs and then the rows below it in
+ * one launch (1), or every tile in one launch (2).  column_max, fuse_max and column_merge change
  * only the launches, never the results.  The table is process-wide, read at every
  * factorisation and guarded by a mutex; probit_amd.autotune fills it from
  * timed trials and persists it. */

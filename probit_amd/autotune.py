@@ -34,12 +34,13 @@ from . import _lib
 __all__ = ["CANDIDATES", "PACKAGE_TABLE", "device_arch", "get", "set_entry", "table", "save",
            "load", "tune"]
 
-CANDIDATES = {"superstep": (2, 3, 4, 6, 8, 12, 16), "column_max": (0, 256, 512, 1024, 4096),
-              "fuse_max": (0, 512), "column_merge": (0, 1)}
+CANDIDATES = {"superstep": (2, 3, 4, 6, 8, 12, 16),
+              "column_max": (0, 256, 512, 1024, 2048, 4096, 16384),
+              "fuse_max": (0, 512), "column_merge": (0, 1, 2)}
 PACKAGE_TABLE = Path(__file__).resolve().parent / "tune_table.json"
 _FIELDS = ("superstep", "column_max", "fuse_max", "column_merge")
 # a persisted entry without a field takes this value
-_DEFAULTS = {"column_merge": 1}
+_DEFAULTS = {"column_merge": 2}
 _PRECISIONS = ("fp64", "fp32", "bf16")
 
 

@@ -426,6 +426,10 @@ static void newton_iter(const Batch& all, const Batch* fac, const Batch* fac32,
                         hipStream_t s) {
   launch_gemv_sym(all, V_Z, V_F, 0, s);
   launch_lik_newton(all, s);
+  // the members that re-factor B solve with the new W^1/2; the others keep
+  // the one their factor was formed with (a chord step)
+  if (fac && fac->count > 0) launch_vec_op(*fac, OP_SNAP, s);
+  if (fac32 && fac32->count > 0) launch_vec_op(*fac32, OP_SNAP, s);
   if (fac && fac->count > 0) factor(*fac, prec, 0, s, V_R);
   if (fac32 && fac32->count > 0) factor(*fac32, PGP_PREC_FP32, 0, s, V_R);
   if (nofac && nofac->count > 0) launch_symv_tiles(*nofac, V_R, s);
@@ -498,6 +502,43 @@ static int default_refine(int prec) { return prec == PGP_PREC_BF16 ? 2 : 0; }
 // step norm above half the previous one.
 static bool stalled(int iteration, double step, double prev) {
   return iteration >= 3 && step > 0.5 * prev;
+}
+
+// Newton iterations of the FP32 mode re-use the factor of B once the step
+// norm has fallen to CHORD_STEP (a chord step: B and its W^1/2 from the last
+// factorisation, the residual at the current weight in FP64, so the fixed
+// point is the same); a chord step that contracts by less than half, or a
+// larger step, re-factors at the next iteration.  The FP64 and BF16 modes
+// re-factor at every iteration of a non-Gaussian likelihood (the
+// reference's Newton, probit/implicit/solvers.py:18-25), a Gaussian one only
+// at the first (W is constant).
+constexpr double CHORD_STEP = 0.1;
+// A run of chord steps converges linearly, so it stops at a tenth of the
+// tolerance (a Newton step's error is quadratic in its step norm).
+constexpr double CHORD_TOL = 0.1;
+struct NewtonTrack {
+  double prev = INFINITY, prev2 = INFINITY;  // the last two step norms
+  bool chord = false;                        // the last iteration re-used the factor
+};
+static bool refactor_now(int prec, bool gauss, int it, bool force, const NewtonTrack& t) {
+  if (it == 0 || force) return true;
+  if (gauss) return false;
+  if (prec != PGP_PREC_FP32) return true;
+  if (!(t.prev <= CHORD_STEP)) return true;
+  return t.chord && t.prev > 0.5 * t.prev2;
+}
+// A chord step: an FP32-mode iteration of a non-Gaussian likelihood that
+// re-used the factor (a Gaussian likelihood's W never changes).
+static bool chord_step(int prec, bool gauss, bool refactored) {
+  return !refactored && !gauss && prec == PGP_PREC_FP32;
+}
+static void track_step(NewtonTrack& t, double step, bool chord) {
+  t.prev2 = t.prev;
+  t.prev = step;
+  t.chord = chord;
+}
+static bool newton_done(double step, double tol, const NewtonTrack& t) {
+  return step <= (t.chord ? CHORD_TOL * tol : tol);
 }
 
 // ---------------------------------------------------------------------------
@@ -604,8 +645,8 @@ int pgp_tune_set(int32_t precision, int32_t size_class, const pgp_tune_entry_t* 
     return fail(-3, "superstep must be in [1, 16] (got %d)", entry->superstep);
   if (entry->column_max < 0 || entry->fuse_max < 0)
     return fail(-3, "column_max / fuse_max must be >= 0");
-  if (entry->column_merge != 0 && entry->column_merge != 1)
-    return fail(-3, "column_merge must be 0 or 1 (got %d)", entry->column_merge);
+  if (entry->column_merge < 0 || entry->column_merge > 2)
+    return fail(-3, "column_merge must be 0, 1 or 2 (got %d)", entry->column_merge);
   tune_set(precision, size_class, *entry);
   return 0;
 }
@@ -895,8 +936,10 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
   const bool gauss = lik->kind == PGP_LIK_GAUSSIAN;
   int cur = prec;  // BF16 escalates to FP32 (same storage) if its factor fails
   bool force_factor = false;
-  while (step > opts->tolerance && it < opts->maxiter) {
-    const bool refactor = !gauss || it == 0 || force_factor;
+  NewtonTrack track;
+  while (!newton_done(step, opts->tolerance, track) && it < opts->maxiter) {
+    const bool refactor = refactor_now(cur, gauss, it, force_factor, track);
+    const bool chord = chord_step(cur, gauss, refactor);
     force_factor = false;
     hipGraphExec_t gexec = nullptr;
     if (use_graphs)
@@ -934,6 +977,7 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
       force_factor = true;
     }
     step = st;
+    track_step(track, st, chord);
     ++it;
     if (!(step == step) || (inf0 & PGP_INFO_NOT_PD)) break;  // stop, report through status
   }
@@ -1491,7 +1535,7 @@ namespace pgp {
 // and 4, 8 the Newton members not re-factored.  Arrays 3, 4, 5 and 7 belong
 // to the lane's finalisation stream.
 constexpr int BATCH_ARRAYS = 9;
-constexpr int NLANES = 2;
+constexpr int NLANES = 4;
 
 struct BItem {
   pgp_batch_item_t* item;
@@ -1508,6 +1552,8 @@ struct BItem {
   bool clear = false;  // its NOT_PD flag is to be cleared before the next step
   bool refac = false;  // re-factor at the next step (escalated Gaussian member)
   double prev = INFINITY;  // step norm of the previous iteration
+  NewtonTrack track;       // re-factor rule (refactor_now)
+  bool chord = false;      // this member's current iteration is a chord step
 };
 
 struct Lane {
@@ -1740,7 +1786,8 @@ static void clear_escalated(Lane& L, const std::vector<BItem*>& v) {
 }
 
 static bool converged(const pgp_batcher* bt, const BItem* b) {
-  return b->step <= bt->opts.tolerance || b->iter >= bt->opts.maxiter || !(b->step == b->step);
+  return newton_done(b->step, bt->opts.tolerance, b->track) || b->iter >= bt->opts.maxiter ||
+         !(b->step == b->step);
 }
 
 // Step norm of a member after its Newton iteration; true if it converged.
@@ -1749,7 +1796,26 @@ static bool advance(pgp_batcher* bt, BItem* b, double h2) {
   if (escalate(bt, b)) return false;
   b->iter++;
   b->prev = b->step;
+  track_step(b->track, b->step, b->chord);
   return converged(bt, b);
+}
+
+// Factor lists of one Newton step (refactor_now per member; escalated BF16
+// members factor with FP32 updates).
+static void split_factor(const pgp_batcher* bt, const std::vector<BItem*>& act,
+                         std::vector<BItem*>& fac, std::vector<BItem*>& fac32,
+                         std::vector<BItem*>& nofac) {
+  const int prec = bt->opts.precision;
+  for (BItem* b : act) {
+    const int cur = b->esc ? PGP_PREC_FP32 : prec;
+    const bool f = refactor_now(cur, b->gauss, b->iter, b->refac, b->track);
+    b->chord = chord_step(cur, b->gauss, f);
+    if (f)
+      (b->esc ? fac32 : fac).push_back(b);
+    else
+      nofac.push_back(b);
+    b->refac = false;
+  }
 }
 
 // Enqueue one step of lane `li` (finalisation of the members that converged
@@ -1785,13 +1851,7 @@ static int lane_launch(pgp_batcher* bt, int li) {
   std::stable_sort(L.act_list.begin(), L.act_list.end(),
                    [](const BItem* a, const BItem* b) { return a->p.Np > b->p.Np; });
   std::vector<BItem*> fac, fac32, nofac;
-  for (BItem* b : L.act_list) {
-    if (!b->gauss || b->iter == 0 || b->refac)
-      (b->esc ? fac32 : fac).push_back(b);
-    else
-      nofac.push_back(b);
-    b->refac = false;
-  }
+  split_factor(bt, L.act_list, fac, fac32, nofac);
   const bool fin = !L.fin_list.empty();
   if (fin) {
     // s2 follows everything enqueued on s so far, then runs beside the
@@ -1973,13 +2033,7 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   if (rc) return rc;
   for (BItem& b : bt->items) active.push_back(&b);
   clear_escalated(L, active);
-  for (BItem* b : active) {
-    if (!b->gauss || b->iter == 0 || b->refac)
-      (b->esc ? fac32 : fac).push_back(b);
-    else
-      nofac.push_back(b);
-    b->refac = false;
-  }
+  split_factor(bt, active, fac, fac32, nofac);
   const Batch ba = make_batch(L, 1, active);
   const Batch bf = make_batch(L, 2, fac);
   const Batch bf32 = make_batch(L, 6, fac32);

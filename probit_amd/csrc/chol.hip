@@ -709,9 +709,11 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
         L[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(c0 + acc_col(bn)) * P.Np] = acc[bm][bn][r];
 }
 
-// ---- one launch per super-step (merged column kernel) ------------------------
-// The columns [s0, s1) of a super-step in one launch: every tile (i, j) of
-// those columns (i >= j) is one workgroup, in ticket order column by column
+// ---- column-form super-step in two launches ----------------------------------
+// Phase A (k_superstep): the tiles (i, j) of the super-step's columns
+// [s0, s1) with row block i < rlim (the super-step's diagonal block of
+// tiles when rlim = s0 + G) in one launch, each tile one workgroup, in
+// ticket order column by column
 // (column j: the diagonal workgroups of the members with more than j block
 // columns, then their row workgroups).  A workgroup applies the left-looking
 // update by the columns s0..j-1 as k_column does, fetching those columns'
@@ -721,8 +723,11 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
 // flag is published after its stores complete.  The chain from one column to
 // the next is tile (j+1, j) -> block j+1, so a column's factor overlaps the
 // previous column's remaining rows, and the G - 1 launch gaps of the
-// per-column form go away.  Same operations in the same order on every tile
-// as k_column: the factor is bitwise the same.
+// per-column form go away.  Phase B (k_panel_rows) takes the rows below,
+// i >= rlim: one workgroup per row block, which forms its tiles (i, s0),
+// (i, s0 + 1), ... in order from its own earlier tiles and phase A's
+// (launch-ordered, so plain cached loads and no waits).  Same operations in
+// the same order on every tile as k_column: the factor is bitwise the same.
 //
 // Member sync words (Prob::csync, zeroed when the descriptor is written with
 // reset): flag (i, c) at csync[(c - s0) nb + i] holds the launch's epoch
@@ -734,9 +739,10 @@ constexpr int MAXG = 16;
 struct StepSched {
   int32_t count;             // members (sorted by size, descending)
   int32_t s0, s1;            // columns [s0, s1)
+  int32_t rlim;              // row blocks [j, min(nb, rlim)) of each column j
   int32_t colbase[MAXG + 1];  // first ticket of column s0 + c
   int32_t nd[MAXG];           // members with more than s0 + c block columns
-  int32_t pre[MAXB + 1];      // pre[m]: block columns of the members before m
+  int32_t pre[MAXB + 1];      // pre[m]: sum of min(nb, rlim) over the members before m
 };
 
 __device__ __forceinline__ uint32_t* tile_flag(const Prob& P, int nb, int s0, int i, int c) {
@@ -787,8 +793,8 @@ __global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, Step
     mi = q;
     i = j;
   } else {
-    // row workgroups of column j: member m has nb_m - j - 1, so the members
-    // before m hold pre[m] - (j + 1) m of them
+    // row workgroups of column j: member m has min(nb_m, rlim) - j - 1, so
+    // the members before m hold pre[m] - (j + 1) m of them
     q -= ss.nd[jj];
     int lo = 0, hi = ss.nd[jj];
     while (hi - lo > 1) {
@@ -853,8 +859,8 @@ __global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, Step
     __hip_atomic_store(tile_flag(P, nb, s0, i, j), epoch, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
     // the member's last workgroup in this launch clears its sync words
-    const int e = min(ss.s1, nb);
-    const uint32_t total = (uint32_t)((e - s0) * nb - (s0 + e - 1) * (e - s0) / 2);
+    const int e = min(ss.s1, nb), nbr = min(nb, ss.rlim);
+    const uint32_t total = (uint32_t)((e - s0) * nbr - (s0 + e - 1) * (e - s0) / 2);
     uint32_t* done = P.csync + (size_t)MAXG * nb;
     s_val = (atomicAdd(done, 1u) + 1u == total) ? 1 : 0;
   }
@@ -957,8 +963,10 @@ __global__ __launch_bounds__(NTHREADS) void k_dinv_raw(const T* A, int64_t ld, d
 // first column has at most this many workgroups run in the column kernel;
 // 0 disables), FUSE_MAX (update launches of at most this many tiles factor
 // the next diagonal block inside the launch; 0 disables) and COLUMN_MERGE
-// (1: such a super-step is one k_superstep launch, 0: one k_column launch
-// per column).
+// (how such a super-step is launched: 0 one k_column launch per column;
+// 1 the diagonal block of tiles in one k_superstep launch, then the rows
+// below in one k_panel_rows launch; 2 every tile in one k_superstep
+// launch).
 // Size class 0 holds the members with padded N <= TUNE_CLASS_NP, class 1 the
 // larger ones; a batch is factored class by class, so a member's schedule --
 // and with G its rounding -- depends only on its own size (a result is the
@@ -966,7 +974,7 @@ __global__ __launch_bounds__(NTHREADS) void k_dinv_raw(const T* A, int64_t ld, d
 // COLUMN_MAX, FUSE_MAX and COLUMN_MERGE change only the launches, never the
 // arithmetic.
 // Initial values: PGP_SUPERSTEP / PGP_COLUMN_MAX / PGP_FUSE_MAX /
-// PGP_COLUMN_MERGE from the environment, else 8 / 512 / 512 / 1.  Read at every factorisation, written
+// PGP_COLUMN_MERGE from the environment, else 8 / 512 / 512 / 2.  Read at every factorisation, written
 // rarely (pgp_tune_set): a mutex guards it.
 // ---------------------------------------------------------------------------
 constexpr int TUNE_CLASSES = 2;
@@ -983,7 +991,7 @@ struct TuneTable {
   TuneTable() {
     const int G = std::min(16, std::max(1, env_int("PGP_SUPERSTEP", 8)));
     const int cm = env_int("PGP_COLUMN_MAX", 512), fm = env_int("PGP_FUSE_MAX", 512);
-    const int mg = env_int("PGP_COLUMN_MERGE", 1) ? 1 : 0;
+    const int mg = std::min(2, std::max(0, env_int("PGP_COLUMN_MERGE", 2)));
     for (int p = 0; p < 3; ++p)
       for (int c = 0; c < TUNE_CLASSES; ++c) e[p][c] = pgp_tune_entry_t{G, cm, fm, mg};
   }
@@ -1042,8 +1050,81 @@ static void column_launch(const Batch& b, int j, int s0, bool tpa, hipStream_t s
   }
 }
 
+// Phase B: row block i = s1 + local of a member with more than s1 block
+// columns, columns [s0, s1): L_ij = (A_ij - sum_{c in [s0, j)} L_ic L_jc^T)
+// DINV_j^T, j in order.  L_jc and DINV_j come from phase A (the previous
+// launch), L_ic from this workgroup's earlier iterations.
 template <typename T, int BF16>
-static void superstep_launch(const Batch& b, int s0, int s1, bool tpa, hipStream_t s) {
+struct RowsArena {
+  static constexpr int PP = KP<T>::value;
+  static constexpr int OPND = BF16 ? 72 * 2 : (MmajorF32<T, BF16>::value ? 68 * 4 : PP * (int)sizeof(T));
+  static constexpr int A = 2 * TILE * OPND;                   // update operands
+  static constexpr int B = 2 * TILE * PP * (int)sizeof(T);  // C_ij and DINV_j
+  static constexpr int bytes = A > B ? A : B;
+};
+
+template <typename T, int BF16>
+__global__ __launch_bounds__(NTHREADS) void k_panel_rows(Prob* const* probs, Sched sc, int s0,
+                                                         int s1) {
+  constexpr int PP = KP<T>::value;
+  __shared__ __attribute__((aligned(16))) char arena[RowsArena<T, BF16>::bytes];
+  int local;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, local)];
+  const int i = s1 + local;
+  if (i >= P.Np / TILE) return;
+  const int64_t ri = (int64_t)i * TILE;
+  T* ck = (T*)arena;
+  T* bk = ck + TILE * PP;
+  T* L = (T*)P.L;
+  const int n = threadIdx.x & 63, pb = threadIdx.x >> 6;
+  for (int j = s0; j < s1; ++j) {
+    const int64_t c0 = (int64_t)j * TILE;
+    update_tile_to_lds<T, BF16, T>((const T*)P.L, P.Np, s0, j, ri, c0, arena, ck, PP, true);
+    const double* Dj = P.dinv + (size_t)j * TILE * TILE;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int pp = pb + 4 * q;
+      bk[pp * PP + n] = (T)Dj[n + pp * TILE];
+    }
+    __syncthreads();
+    typename Acc<T>::type acc[2][2];
+    zero_acc<T>(acc);
+    mma64_kmajor<T>(ck, PP, bk, PP, acc);
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          L[(size_t)(ri + acc_row<T>(bm, r)) + (size_t)(c0 + acc_col(bn)) * P.Np] =
+              acc[bm][bn][r];
+    // the tile is an operand of this row's next columns; LDS is free again
+    publish_wait();
+    __syncthreads();
+  }
+}
+
+template <typename T, int BF16>
+static void rows_launch(const Batch& b, int s0, int s1, bool tpa, hipStream_t s) {
+  constexpr double G64 = 2.0 * TILE * TILE * TILE;
+  sched_launch(
+      b, [&](int m) { return std::max(0, b.np(m) / TILE - s1); },
+      [&](Prob* const* dp, const Sched& sc, int n) {
+        if (tpa) timer_begin(TC_PANEL, s);
+        hipLaunchKernelGGL((k_panel_rows<T, BF16>), dim3(n), dim3(NTHREADS), 0, s, dp, sc, s0, s1);
+        if (tpa) {
+          const int off = (int)(dp - b.dprobs);
+          double fl = 0.0;
+          const double per_row = G64 * (double)(s1 - s0) * (s1 - s0 + 1) / 2.0;
+          for (int m = off; m < off + sc.count; ++m)
+            fl += std::max(0, b.np(m) / TILE - s1) * per_row;
+          timer_end(TC_PANEL, s, fl, 0.0);
+        }
+      });
+}
+
+template <typename T, int BF16>
+static void superstep_launch(const Batch& b, int s0, int s1, int rlim, bool tpa, hipStream_t s) {
   constexpr double F3 = TILE * TILE * TILE / 3.0;
   constexpr double G64 = 2.0 * TILE * TILE * TILE;
   for (int off = 0; off < b.count; off += MAXB) {
@@ -1052,18 +1133,19 @@ static void superstep_launch(const Batch& b, int s0, int s1, bool tpa, hipStream
     ss.count = n;
     ss.s0 = s0;
     ss.s1 = s1;
+    ss.rlim = rlim;
     ss.pre[0] = 0;
-    for (int m = 0; m < n; ++m) ss.pre[m + 1] = ss.pre[m] + b.np(off + m) / TILE;
+    for (int m = 0; m < n; ++m) ss.pre[m + 1] = ss.pre[m] + std::min(b.np(off + m) / TILE, rlim);
     ss.colbase[0] = 0;
     double fl = 0.0;
     for (int j = s0; j < s1; ++j) {
       int nd = 0, rows = 0;
       for (int m = 0; m < n; ++m) {
-        const int nbm = b.np(off + m) / TILE;
+        const int nbm = b.np(off + m) / TILE, nbr = std::min(nbm, rlim);
         if (nbm > j) {
           ++nd;
-          rows += nbm - j - 1;
-          fl += F3 + (double)(nbm - j) * G64 * (j - s0) + (double)(nbm - j - 1) * G64;
+          rows += nbr - j - 1;
+          fl += F3 + (double)(nbr - j) * G64 * (j - s0) + (double)(nbr - j - 1) * G64;
         }
       }
       ss.nd[j - s0] = nd;
@@ -1158,8 +1240,14 @@ static void potrf_steps(const Batch& b, hipStream_t s, const pgp_tune_entry_t& t
     const int s0 = k / G * G, s_end = s0 + G;
     const bool outer = (k + 1 == s_end);
     if (col_style(s0)) {
-      if (te.column_merge) {
-        if (k == s0) superstep_launch<T, BF16>(b, s0, std::min(s_end, nb), tpa, s);
+      if (te.column_merge == 2) {
+        // every row block of the super-step's columns in the one launch
+        if (k == s0) superstep_launch<T, BF16>(b, s0, std::min(s_end, nb), nb, tpa, s);
+      } else if (te.column_merge == 1) {
+        if (k == s0) {
+          superstep_launch<T, BF16>(b, s0, std::min(s_end, nb), s_end, tpa, s);
+          if (s_end < nb) rows_launch<T, BF16>(b, s0, s_end, tpa, s);
+        }
       } else {
         column_launch<T, BF16>(b, k, s0, tpa, s);
       }

@@ -38,6 +38,7 @@ enum Vec {
   V_X2,     // scratch
   V_X3,     // scratch
   V_X4,     // scratch
+  V_SF,     // sqrt(max(W, 0)) at the last factorisation of B (the Newton solve's)
   NVEC
 };
 
@@ -336,6 +337,7 @@ enum VecOp {
   OP_VBG_Z = 13,     // X4 = S .* U
   OP_VBG_V = 14,     // X4 = (X2 - X3) / sigma^2
   OP_VBG_SHAT = 15,  // X3 = Z - G - sigma W .* X4, X1 = 0, X2 = sigma X4
+  OP_SNAP = 16,      // SF = S: the W^1/2 of the factor B is about to be formed with
 };
 enum RedWhat { RED_STEP = 0, RED_FINAL = 1 };
 void launch_gemv_sym(const Batch& b, int x_slot, int out_slot, int mode, hipStream_t s);

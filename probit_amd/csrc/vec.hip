@@ -22,7 +22,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // mode 0: out = Kx
-// mode 1: out = S .* Kx
+// mode 1: out = SF .* Kx              (SF: the W^1/2 of the factored B)
 // mode 2: out = R - DL - S^2 .* Kx     (Laplace residual of (I + W K) d = r, x = DL)
 // mode 3: out = X1 - sigma^2 x - Kx    (VB residual of (sigma^2 I + K) w = rhs, rhs in X1)
 // (modes >= 10: the same with the matrix C in place of K)
@@ -34,7 +34,7 @@ __device__ __forceinline__ void gemv_epilogue(const Prob& P, int i, const double
       v = sum;
       break;
     case 1:
-      v = vslot(P, V_S)[i] * sum;
+      v = vslot(P, V_SF)[i] * sum;
       break;
     case 2: {
       const double si = vslot(P, V_S)[i];
@@ -201,10 +201,13 @@ __global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, Sched sc, 
   if (i >= P.Np) return;
   switch (op) {
     case OP_STEP:
-      vslot(P, V_DL)[i] = vslot(P, V_R)[i] - vslot(P, V_S)[i] * vslot(P, V_U)[i];
+      vslot(P, V_DL)[i] = vslot(P, V_R)[i] - vslot(P, V_SF)[i] * vslot(P, V_U)[i];
       break;
     case OP_IRCORR:
-      vslot(P, V_DL)[i] += vslot(P, V_Q)[i] - vslot(P, V_S)[i] * vslot(P, V_U)[i];
+      vslot(P, V_DL)[i] += vslot(P, V_Q)[i] - vslot(P, V_SF)[i] * vslot(P, V_U)[i];
+      break;
+    case OP_SNAP:
+      vslot(P, V_SF)[i] = vslot(P, V_S)[i];
       break;
     case OP_ZUPD:  // a step from a failed factor is not taken
       if (!(P.info[0] & PGP_INFO_NOT_PD)) vslot(P, V_Z)[i] += vslot(P, V_DL)[i];

@@ -123,9 +123,11 @@ def test_tuning_run_persists_a_table_that_loads_back(restore_table, tmp_path):
 
 @pytest.mark.parametrize("mode", ["fp64", "fp32", "bf16"])
 def test_merged_superstep_kernel_equals_per_column_launches(restore_table, mode):
-    """One k_superstep launch per column-form super-step against one k_column
-    launch per block column: the same factor bits, single fits (Gaussian and
-    probit) and a batcher mix of sizes in both classes, with fewer launches."""
+    """The two merged forms of a column-form super-step (its diagonal block in
+    one k_superstep launch and the rows below in one k_panel_rows launch; all
+    of it in one k_superstep launch) against one k_column launch per block
+    column: the same factor bits, single fits (Gaussian and probit) and a
+    batcher mix of sizes in both classes, with fewer launches."""
     import torch
 
     from probit_amd import _lib
@@ -158,18 +160,19 @@ def test_merged_superstep_kernel_equals_per_column_launches(restore_table, mode)
         return single, out, launches
 
     print(f"merged superstep {mode}", flush=True)
-    s1, b1, n1 = run(1)
     s0, b0, n0 = run(0)
-    assert n1 < n0
-    for (w1, it1), (w0, it0) in zip(s1, s0):
-        assert np.array_equal(w1, w0) and it1 == it0
-    assert sorted(b1) == sorted(b0) == list(range(len(sizes)))
-    for t in b1:
-        for name in ("mean", "var", "weight", "precision"):
-            a, b = getattr(b1[t], name), getattr(b0[t], name)
-            if a is None:
-                assert b is None
-                continue
-            assert torch.equal(a, b), (mode, t, sizes[t], name)
-        assert b1[t].iterations == b0[t].iterations and b1[t].status == b0[t].status
-        assert b1[t].status & ~_lib.INFO_ESCALATED == 0, (t, b1[t].status)
+    for merge in (1, 2):
+        s1, b1, n1 = run(merge)
+        assert n1 < n0, (merge, n1, n0)
+        for (w1, it1), (w0, it0) in zip(s1, s0):
+            assert np.array_equal(w1, w0) and it1 == it0, merge
+        assert sorted(b1) == sorted(b0) == list(range(len(sizes)))
+        for t in b1:
+            for name in ("mean", "var", "weight", "precision"):
+                a, b = getattr(b1[t], name), getattr(b0[t], name)
+                if a is None:
+                    assert b is None
+                    continue
+                assert torch.equal(a, b), (mode, merge, t, sizes[t], name)
+            assert b1[t].iterations == b0[t].iterations and b1[t].status == b0[t].status
+            assert b1[t].status & ~_lib.INFO_ESCALATED == 0, (t, b1[t].status)
