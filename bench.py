@@ -38,6 +38,14 @@ import argparse
 import json
 import math
 import os
+
+# The serving engine runs each size lane on two streams of its own beside the
+# caller's; the HIP runtime maps a process's streams onto GPU_MAX_HW_QUEUES
+# hardware queues (4 by default), and streams that share a queue serialise.
+# The runtime reads it when it starts, so it is raised to 8 before torch is
+# imported.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 import sys
 import time
 
@@ -497,6 +505,7 @@ def main():
             "requests_per_step": R,
             "capacity": args.capacity,
             "lane_np": os.environ.get("PGP_LANE_NP", "1024"),
+            "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
             "min_n": args.min_n,
             "precision": args.precision,
             "tolerance": 1e-5,

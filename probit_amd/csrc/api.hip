@@ -416,15 +416,17 @@ static void solve_B(const Batch& b, int prec, hipStream_t s) {
 }
 
 // One Newton iteration z <- z + (I + W K)^-1 (grad - z) for every member of
-// `all`; members of `fac` re-form and re-factor B first (those of `fac32`
+// `all` (`later`: those past their first iteration); members of `fac`
+// re-form and re-factor B first (those of `fac32`
 // with FP32 updates: BF16 members escalated after a loss of definiteness;
 // both store the factor in FP32), and `nofac` lists the others.  The pass
 // that forms B also takes the partial sums of K r, so only the members of
 // `nofac` read K once more for it.  ||delta||^2 goes to gather[b].
-static void newton_iter(const Batch& all, const Batch* fac, const Batch* fac32,
-                        const Batch* nofac, int prec, int refine, double* gather,
-                        hipStream_t s) {
-  launch_gemv_sym(all, V_Z, V_F, 0, s);
+static void newton_iter(const Batch& all, const Batch* later, const Batch* fac,
+                        const Batch* fac32, const Batch* nofac, int prec, int refine,
+                        double* gather, hipStream_t s) {
+  // f = K z; at a member's first iteration z = 0 and f was zeroed with it
+  if (later && later->count > 0) launch_gemv_sym(*later, V_Z, V_F, 0, s);
   launch_lik_newton(all, s);
   // the members that re-factor B solve with the new W^1/2; the others keep
   // the one their factor was formed with (a chord step)
@@ -803,12 +805,13 @@ int64_t pgp_grad_workspace_size(int64_t N, int32_t D) {
 struct GraphKey {
   int dev;
   const char* ws;
-  int Np, prec, refactor, refine;
+  int Np, prec, refactor, refine, first;
   pgp_tune_entry_t te;  // the schedule the factor's launches follow
   bool operator==(const GraphKey& o) const {
     return dev == o.dev && ws == o.ws && Np == o.Np && prec == o.prec &&
-           refactor == o.refactor && refine == o.refine && te.superstep == o.te.superstep &&
-           te.column_max == o.te.column_max && te.fuse_max == o.te.fuse_max;
+           refactor == o.refactor && refine == o.refine && first == o.first &&
+           te.superstep == o.te.superstep && te.column_max == o.te.column_max &&
+           te.fuse_max == o.te.fuse_max && te.column_merge == o.te.column_merge;
   }
 };
 struct GraphEntry {
@@ -876,8 +879,8 @@ static hipGraphExec_t newton_graph(const GraphKey& k, int fit, const Batch& b, i
     return nullptr;
   hipStream_t cs = gc.capture[k.dev];
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
-  newton_iter(b, k.refactor ? &b : nullptr, nullptr, k.refactor ? nullptr : &b, prec, refine,
-              gather, cs);
+  newton_iter(b, k.first ? nullptr : &b, k.refactor ? &b : nullptr, nullptr,
+              k.refactor ? nullptr : &b, prec, refine, gather, cs);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(cs, &g);
   hipGraphExec_t exec = nullptr;
@@ -944,15 +947,15 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
     hipGraphExec_t gexec = nullptr;
     if (use_graphs)
       gexec = newton_graph(GraphKey{dev, ws, (int)p.Np, cur, refactor ? 1 : 0, refine,
-                                    tune_get(cur, tune_class_of((int)p.Np))},
+                                    it == 0 ? 1 : 0, tune_get(cur, tune_class_of((int)p.Np))},
                            fit_id, b, cur, refine, gather);
     if (gexec) {
       HIP_OK(hipGraphLaunch(gexec, s));
       ++g_graph_launches;
     } else {
       ++g_direct_iters;
-      newton_iter(b, refactor ? &b : nullptr, nullptr, refactor ? nullptr : &b, cur, refine,
-                  gather, s);
+      newton_iter(b, it == 0 ? nullptr : &b, refactor ? &b : nullptr, nullptr,
+                  refactor ? nullptr : &b, cur, refine, gather, s);
     }
     rc = check_launch();
     if (rc) return rc;
@@ -1534,7 +1537,7 @@ namespace pgp {
 // ones, 5 predicted ones, 6 and 7 the FP32-escalated factored members of 2
 // and 4, 8 the Newton members not re-factored.  Arrays 3, 4, 5 and 7 belong
 // to the lane's finalisation stream.
-constexpr int BATCH_ARRAYS = 9;
+constexpr int BATCH_ARRAYS = 10;
 constexpr int NLANES = 4;
 
 struct BItem {
@@ -1864,11 +1867,15 @@ static int lane_launch(pgp_batcher* bt, int li) {
     for (BItem* b : L.fin_list) b->phase = 3;
   }
   if (!L.act_list.empty()) {
+    std::vector<BItem*> later;
+    for (BItem* b : L.act_list)
+      if (b->iter > 0) later.push_back(b);
     const Batch ba = make_batch(L, 1, L.act_list);
+    const Batch bl = make_batch(L, 9, later);
     const Batch bf = make_batch(L, 2, fac);
     const Batch bf32 = make_batch(L, 6, fac32);
     const Batch bn = make_batch(L, 8, nofac);
-    newton_iter(ba, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
+    newton_iter(ba, &bl, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
     HIP_OK(hipMemcpyAsync(L.hstep, L.gather, L.act_list.size() * 8, hipMemcpyDeviceToHost, s));
   }
   rc = check_launch();
@@ -2034,11 +2041,15 @@ int pgp_batcher_step(pgp_batcher_t* bt, int64_t* done_tags, int32_t max_done, in
   for (BItem& b : bt->items) active.push_back(&b);
   clear_escalated(L, active);
   split_factor(bt, active, fac, fac32, nofac);
+  std::vector<BItem*> later;
+  for (BItem* b : active)
+    if (b->iter > 0) later.push_back(b);
   const Batch ba = make_batch(L, 1, active);
+  const Batch bl = make_batch(L, 9, later);
   const Batch bf = make_batch(L, 2, fac);
   const Batch bf32 = make_batch(L, 6, fac32);
   const Batch bn = make_batch(L, 8, nofac);
-  newton_iter(ba, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
+  newton_iter(ba, &bl, &bf, &bf32, &bn, prec, bt->refine, L.gather, s);
   rc = check_launch();
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(L.hstep, L.gather, active.size() * 8, hipMemcpyDeviceToHost, s));

@@ -212,8 +212,9 @@ __global__ __launch_bounds__(NTHREADS) void k_vec(Prob* const* probs, Sched sc, 
     case OP_ZUPD:  // a step from a failed factor is not taken
       if (!(P.info[0] & PGP_INFO_NOT_PD)) vslot(P, V_Z)[i] += vslot(P, V_DL)[i];
       break;
-    case OP_ZERO_Z:
+    case OP_ZERO_Z:  // the zero weight and its latent f = K 0
       vslot(P, V_Z)[i] = 0.0;
+      vslot(P, V_F)[i] = 0.0;
       break;
     case OP_OUT:
       if (i < P.N) {

@@ -109,11 +109,11 @@ def test_tuning_run_persists_a_table_that_loads_back(restore_table, tmp_path):
     path = tmp_path / "tune.json"
     res = at.tune(precisions=("fp32",), classes=(0,),
                   candidates={"superstep": (4, 8), "column_max": (0, 512), "fuse_max": (512,),
-                              "column_merge": (1,)},
+                              "column_merge": (2,)},
                   window=24, capacity=16, reps=1, path=path)
     best = res["fp32"][0]["entry"]
     assert best["superstep"] in (4, 8) and best["column_max"] in (0, 512)
-    assert best["column_merge"] == 1
+    assert best["column_merge"] == 2
     assert len(res["fp32"][0]["trials"]) >= 3
     assert at.get("fp32", 0) == best
     at.set_entry("fp32", 0, {"superstep": 2, "column_max": 7, "fuse_max": 9, "column_merge": 0})
