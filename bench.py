@@ -84,7 +84,7 @@ def parse():
                    help="requests of the open-loop (Poisson arrivals) latency run")
     p.add_argument("--rate", type=float, default=0.0,
                    help="open-loop arrival rate (req/s); 0 = 0.7 x measured throughput")
-    p.add_argument("--cpu-strata", type=int, default=12,
+    p.add_argument("--cpu-strata", type=int, default=48,
                    help="CPU baseline: requests timed, one per equal-count N-stratum")
     p.add_argument("--cpu-study", type=int, default=0, metavar="REPEATS",
                    help="only time the CPU baseline plan (both oracle variants, all cores and "
@@ -353,15 +353,31 @@ def main():
     eng = ServingEngine(precision=args.precision, capacity=args.capacity,
                         refine=None if args.refine < 0 else args.refine)
     if args.kernel_window:
+        # every launch-timer class on: HIP events around the launches (no
+        # kernels), so a profiler sees the same launches with or without them
         win = dev_reqs[W * R:W * R + args.kernel_window]
+        lib.pgp_timer_reset()
+        mask = 0
+        for _, attr, _ in TIMER_CLASSES:
+            mask |= 1 << getattr(_lib, attr)
+        lib.pgp_timer_enable(mask)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         done = kernel_window(eng, win)
         torch.cuda.synchronize()
-        print(json.dumps({"kernel_window": {"requests": len(done), "seconds":
-                                            round(time.perf_counter() - t0, 4),
+        secs = time.perf_counter() - t0
+        lib.pgp_timer_enable(0)
+        classes = {}
+        for name, attr, _ in TIMER_CLASSES:
+            n_l, ms, fl, by = read_timer(lib, _lib, getattr(_lib, attr))
+            if n_l:
+                classes[name] = {"launches": n_l, "avg_launch_us": round(1e3 * ms / n_l, 3),
+                                 "algo_flops_per_launch": fl / n_l,
+                                 "algo_bytes_per_launch": by / n_l}
+        print(json.dumps({"kernel_window": {"requests": len(done), "seconds": round(secs, 4),
                                             "precision": args.precision,
-                                            "timed_set": _mix([r["p"] for r in win])}}))
+                                            "timed_set": _mix([r["p"] for r in win]),
+                                            "timer_classes": classes}}))
         return
     # warmup windows
     for s in range(W):
