@@ -713,9 +713,7 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
 // Phase A (k_superstep): the tiles (i, j) of the super-step's columns
 // [s0, s1) with row block i < rlim (the super-step's diagonal block of
 // tiles when rlim = s0 + G) in one launch, each tile one workgroup, in
-// ticket order column by column
-// (column j: the diagonal workgroups of the members with more than j block
-// columns, then their row workgroups).  A workgroup applies the left-looking
+// lookahead ticket order (StepSched).  A workgroup applies the left-looking
 // update by the columns s0..j-1 as k_column does, fetching those columns'
 // tiles -- written in this launch -- with coherent loads once their flags
 // show them final; a diagonal workgroup then factors and inverts block j, a
@@ -736,13 +734,22 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
 // flags, so no flag outlives its launch (a replayed graph of the launch,
 // whose epoch is the captured one, starts from cleared flags).
 constexpr int MAXG = 16;
+// Tickets run in groups k = s0 .. s1 (lookahead order): group k holds the
+// diagonal workgroups D(k) of every member, then the rows i >= k + 1 of
+// column k - 1 (B(k - 1)), then tile (k + 1, k) (the one block k + 1 needs
+// from column k).  So the chain D(k) -> (k + 1, k) -> D(k + 1) is ticketed
+// ahead of the bulk of each column, whose workgroups start once their
+// diagonal block is (nearly) done instead of holding a slot while they
+// wait; every dependency is on a lower ticket.
 struct StepSched {
   int32_t count;             // members (sorted by size, descending)
   int32_t s0, s1;            // columns [s0, s1)
   int32_t rlim;              // row blocks [j, min(nb, rlim)) of each column j
-  int32_t colbase[MAXG + 1];  // first ticket of column s0 + c
-  int32_t nd[MAXG];           // members with more than s0 + c block columns
-  int32_t pre[MAXB + 1];      // pre[m]: sum of min(nb, rlim) over the members before m
+  int32_t gbase[MAXG + 2];   // first ticket of group s0 + g
+  int32_t nd[MAXG + 1];      // D(k): members with more than k block columns
+  int32_t nB[MAXG + 1];      // members with min(nb, rlim) > k + 1 (B(k - 1) and tile (k + 1, k))
+  int32_t nbk[MAXG + 1];     // workgroups of B(k - 1)
+  int32_t pre[MAXB + 1];     // pre[m]: sum of min(nb, rlim) over the members before m
 };
 
 __device__ __forceinline__ uint32_t* tile_flag(const Prob& P, int nb, int s0, int i, int c) {
@@ -783,29 +790,33 @@ __global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, Step
     flag_members_stalled(probs, ss.count);
     return;
   }
-  // ticket -> (column j, member mi, row block i)
-  int jj = 0;
-  while (t >= ss.colbase[jj + 1]) ++jj;
-  const int s0 = ss.s0, j = s0 + jj;
-  int q = t - ss.colbase[jj];
-  int mi, i;
-  if (q < ss.nd[jj]) {
+  // ticket -> (member mi, row block i, column j)
+  int g = 0;
+  while (t >= ss.gbase[g + 1]) ++g;
+  const int s0 = ss.s0, k = s0 + g;
+  int q = t - ss.gbase[g];
+  int mi, i, j;
+  if (q < ss.nd[g]) {  // D(k)
     mi = q;
-    i = j;
-  } else {
-    // row workgroups of column j: member m has min(nb_m, rlim) - j - 1, so
-    // the members before m hold pre[m] - (j + 1) m of them
-    q -= ss.nd[jj];
-    int lo = 0, hi = ss.nd[jj];
+    i = j = k;
+  } else if ((q -= ss.nd[g]) < ss.nbk[g]) {
+    // B(k - 1): member m has min(nb_m, rlim) - k - 1 rows, so the members
+    // before m hold pre[m] - (k + 1) m of them
+    int lo = 0, hi = ss.nB[g];
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
-      if (ss.pre[mid] - (j + 1) * mid <= q)
+      if (ss.pre[mid] - (k + 1) * mid <= q)
         lo = mid;
       else
         hi = mid;
     }
     mi = lo;
-    i = j + 1 + (q - (ss.pre[lo] - (j + 1) * lo));
+    i = k + 1 + (q - (ss.pre[lo] - (k + 1) * lo));
+    j = k - 1;
+  } else {  // tile (k + 1, k)
+    mi = q - ss.nbk[g];
+    i = k + 1;
+    j = k;
   }
   const Prob& P = *probs[mi];
   const int nb = P.Np / TILE;
@@ -1136,22 +1147,31 @@ static void superstep_launch(const Batch& b, int s0, int s1, int rlim, bool tpa,
     ss.rlim = rlim;
     ss.pre[0] = 0;
     for (int m = 0; m < n; ++m) ss.pre[m + 1] = ss.pre[m] + std::min(b.np(off + m) / TILE, rlim);
-    ss.colbase[0] = 0;
     double fl = 0.0;
-    for (int j = s0; j < s1; ++j) {
-      int nd = 0, rows = 0;
+    for (int m = 0; m < n; ++m) {
+      const int nbm = b.np(off + m) / TILE, nbr = std::min(nbm, rlim);
+      for (int j = s0; j < std::min(s1, nbm); ++j)
+        fl += F3 + (double)(nbr - j) * G64 * (j - s0) + (double)(nbr - j - 1) * G64;
+    }
+    const int ncols = s1 - s0;
+    ss.gbase[0] = 0;
+    for (int g = 0; g <= ncols; ++g) {
+      const int k = s0 + g;
+      int nd = 0, nB = 0, nbk = 0;
       for (int m = 0; m < n; ++m) {
         const int nbm = b.np(off + m) / TILE, nbr = std::min(nbm, rlim);
-        if (nbm > j) {
-          ++nd;
-          rows += nbr - j - 1;
-          fl += F3 + (double)(nbr - j) * G64 * (j - s0) + (double)(nbr - j - 1) * G64;
+        if (g < ncols && nbm > k) ++nd;
+        if (nbr > k + 1) {
+          ++nB;
+          if (g >= 1) nbk += nbr - k - 1;
         }
       }
-      ss.nd[j - s0] = nd;
-      ss.colbase[j - s0 + 1] = ss.colbase[j - s0] + nd + rows;
+      ss.nd[g] = nd;
+      ss.nB[g] = nB;
+      ss.nbk[g] = nbk;
+      ss.gbase[g + 1] = ss.gbase[g] + nd + nbk + (g < ncols ? nB : 0);
     }
-    const int tot = ss.colbase[s1 - s0];
+    const int tot = ss.gbase[ncols + 1];
     if (tot == 0) continue;
     const uint32_t ep = next_epoch();
     if (tpa) timer_begin(TC_PANEL, s);
