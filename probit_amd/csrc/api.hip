@@ -514,10 +514,15 @@ static bool stalled(int iteration, double step, double prev) {
 // re-factor at every iteration of a non-Gaussian likelihood (the
 // reference's Newton, probit/implicit/solvers.py:18-25), a Gaussian one only
 // at the first (W is constant).
-constexpr double CHORD_STEP = 0.1;
+static double env_double(const char* name, double dflt) {
+  const char* e = getenv(name);
+  return e ? atof(e) : dflt;
+}
+// (PGP_CHORD_STEP = 0 turns the chord steps off)
+static const double CHORD_STEP = env_double("PGP_CHORD_STEP", 0.1);
 // A run of chord steps converges linearly, so it stops at a tenth of the
 // tolerance (a Newton step's error is quadratic in its step norm).
-constexpr double CHORD_TOL = 0.1;
+static const double CHORD_TOL = env_double("PGP_CHORD_TOL", 0.1);
 struct NewtonTrack {
   double prev = INFINITY, prev2 = INFINITY;  // the last two step norms
   bool chord = false;                        // the last iteration re-used the factor
