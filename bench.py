@@ -289,7 +289,9 @@ def _mix(problems):
             "cbrt_mean_N3": round(float(np.mean(n ** 3) ** (1 / 3)), 1)}
 
 
-TIMER_CLASSES = (("outer_update", "TIMER_SYRK", "mfma"), ("column_steps", "TIMER_PANEL", "mfma"),
+TIMER_CLASSES = (("outer_update", "TIMER_SYRK", "mfma"),
+                 ("merged_column_steps", "TIMER_SUPER", "mfma"),
+                 ("column_steps", "TIMER_PANEL", "mfma"),
                  ("trsv", "TIMER_TRSV", "hbm"), ("gram", "TIMER_GRAM", "hbm"),
                  ("trsm", "TIMER_TRSM", "mfma"), ("variance_refine", "TIMER_REFINE", "mfma"))
 
@@ -384,7 +386,7 @@ def main():
         serve_window(eng, dev_reqs[s * R:(s + 1) * R])
     torch.cuda.synchronize()
     lib.pgp_timer_reset()
-    lib.pgp_timer_enable(1 << _lib.TIMER_SYRK)
+    lib.pgp_timer_enable((1 << _lib.TIMER_SUPER) | (1 << _lib.TIMER_SYRK))
     timed = dev_reqs[W * R:(W + K) * R]
     results = {}
     hs0 = eng.host_stats()
@@ -395,7 +397,7 @@ def main():
     t1 = time.perf_counter()
     hs1 = eng.host_stats()
     lib.pgp_timer_enable(0)
-    n_l, ms, fl, by = read_timer(lib, _lib, _lib.TIMER_SYRK)
+    timed_classes = {c: read_timer(lib, _lib, c) for c in (_lib.TIMER_SUPER, _lib.TIMER_SYRK)}
     elapsed = t1 - t0
     value = K * R / elapsed
     lat_closed = np.array([v[0] for v in results.values()])
@@ -483,24 +485,34 @@ def main():
                       f"reference-faithful LU Newton + dense predict solve (the oracle), NumPy "
                       f"FP64, {sum(t_cpu):.1f} s; posteriors/s = requests / total seconds")
 
-    superstep = int(os.environ.get("PGP_SUPERSTEP", "8"))
-    ms_per_launch = ms / max(1, n_l)
-    achieved = fl / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    # Rooflines over the timed region from the library's HIP-event timers:
+    # the dominant kernel (the merged column kernel of the Cholesky,
+    # k_superstep) and the outer trailing update.  achieved = algorithmic
+    # flops per launch / average launch duration; traffic = this run's
+    # algorithmic bytes per launch x the HBM bytes per algorithmic byte that
+    # the committed FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) measured
+    # on the same kernel (profiles/roofline_traffic.json; the launch mix
+    # varies with the serving dynamics), null if not measured.
     peak = PEAK_TFLOPS[args.precision]
-    # HBM bytes per launch of the outer update: the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated) give
-    # HBM bytes per algorithmic byte over their timed launches
-    # (profiles/syrk_outer_traffic.json); times this run's algorithmic bytes
-    # per launch, since the launch mix varies with the serving dynamics.
-    # null if not measured
-    algo_bytes = by / max(1, n_l)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "syrk_outer_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    ratios = {}
     if os.path.exists(tpath):
         with open(tpath) as f:
-            rec = json.load(f).get(args.precision)
-        if rec and rec.get("hbm_bytes_per_algo_byte"):
-            traffic = round(rec["hbm_bytes_per_algo_byte"] * algo_bytes, -3)
+            ratios = json.load(f).get(args.precision, {})
+
+    def roofline(cls, kernel, label):
+        n_l, ms, fl, by = timed_classes[cls]
+        achieved = fl / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        algo_bytes = by / max(1, n_l)
+        rec = ratios.get(kernel) or {}
+        traffic = (round(rec["hbm_bytes_per_algo_byte"] * algo_bytes, -3)
+                   if rec.get("hbm_bytes_per_algo_byte") else None)
+        return {"kernel": f"{kernel} ({label})", "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
+                "traffic": traffic, "launches": int(n_l),
+                "avg_launch_us": round(1e3 * ms / max(1, n_l), 3),
+                "algo_flops_per_launch": fl / max(1, n_l), "algo_bytes_per_launch": algo_bytes}
+
     line = {
         "metric": "GP posteriors/sec (batched mixed-precision serving, p99 latency reported)",
         "value": round(value, 2),
@@ -553,20 +565,12 @@ def main():
         "speedup_vs_plain_fp64": round(value / plain_rate, 2),
         "max_rel_mean_err_vs_fp64": float(np.max(errs)),
         "mean_newton_iterations": round(float(np.mean(iters)), 2),
-        "roofline": {
-            "kernel": f"k_syrk_outer ({args.precision} rank-{64 * superstep} trailing update of "
-                      "the Cholesky)",
-            "bound": "mfma",
-            "achieved": round(achieved, 3),
-            "peak": peak,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 5),
-            "traffic": traffic,
-            "launches": int(n_l),
-            "avg_launch_us": round(1e3 * ms_per_launch, 3),
-            "algo_flops_per_launch": fl / max(1, n_l),
-            "algo_bytes_per_launch": algo_bytes,
-        },
+        "roofline": roofline(_lib.TIMER_SUPER, "k_superstep",
+                             f"{args.precision} merged column-form super-steps of the "
+                             "Cholesky: left-looking tile updates, FP64 diagonal factors, panels"),
+        "roofline_outer_update": roofline(_lib.TIMER_SYRK, "k_syrk_outer",
+                                          f"{args.precision} rank-64G trailing update of the "
+                                          "Cholesky, G from the schedule table"),
         "kernel_classes": shares,
         "host_loop": {  # the serving loop's host thread over the timed region
             "lane_steps": hs1["lane_steps"] - hs0["lane_steps"],

@@ -151,7 +151,8 @@ int pgp_fit_result_apply_info(pgp_fit_result_t* result, int32_t info_word);
  * kernel class c (0 = outer trailing SYRK update of the Cholesky, 1 = its
  * per-column steps: diagonal factor, panel, column update, 2 = TRSV,
  * 3 = Gram assembly, 4 = TRSM, 5 = the FP64 correction of the predictive
- * variance) are bracketed by HIP events on their stream,
+ * variance, 6 = the merged column kernel of the Cholesky, whose super-steps
+ * class 1 then leaves out) are bracketed by HIP events on their stream,
  * with the algorithmic flops / bytes each launch performs.  pgp_timer_read
  * waits for the recorded events and returns the class totals.  Not
  * thread-safe; no counterpart in the reference (measurement only). */

@@ -42,6 +42,7 @@ TIMER_TRSV = 2
 TIMER_GRAM = 3
 TIMER_TRSM = 4
 TIMER_REFINE = 5
+TIMER_SUPER = 6
 
 LIB_PATH = Path(__file__).resolve().parent / "libprobit_amd.so"
 

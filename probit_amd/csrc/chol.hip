@@ -1174,10 +1174,22 @@ static void superstep_launch(const Batch& b, int s0, int s1, int rlim, bool tpa,
     const int tot = ss.gbase[ncols + 1];
     if (tot == 0) continue;
     const uint32_t ep = next_epoch();
-    if (tpa) timer_begin(TC_PANEL, s);
+    const bool tsu = timer_on(TC_SUPER);
+    double by = 0.0;
+    if (tsu) {
+      // algorithmic bytes: every tile of the launch read and written once in
+      // the storage type, and each diagonal block's FP64 inverse written once
+      constexpr double TB = (double)TILE * TILE;
+      for (int m = 0; m < n; ++m) {
+        const int nbm = b.np(off + m) / TILE, nbr = std::min(nbm, rlim);
+        for (int j = s0; j < std::min(s1, nbm); ++j)
+          by += (double)(nbr - j) * 2.0 * TB * sizeof(T) + TB * 8.0;
+      }
+      timer_begin(TC_SUPER, s);
+    }
     hipLaunchKernelGGL((k_superstep<T, BF16>), dim3(tot), dim3(NTHREADS), 0, s, b.dprobs + off, ss,
                        b.ticket, ep);
-    if (tpa) timer_end(TC_PANEL, s, fl, 0.0);
+    if (tsu) timer_end(TC_SUPER, s, fl, by);
   }
 }
 

@@ -262,7 +262,9 @@ __device__ __forceinline__ int wait_progress(uint64_t* progress, uint32_t epoch,
 //   TC_GRAM  Gram assembly
 //   TC_TRSM  multi-right-hand-side solves
 //   TC_REFINE FP64 variance correction of predict (K U product)
-enum TimerClass { TC_SYRK = 0, TC_PANEL, TC_TRSV, TC_GRAM, TC_TRSM, TC_REFINE, TC_COUNT };
+//   TC_SUPER the merged column kernel (k_superstep)
+enum TimerClass { TC_SYRK = 0, TC_PANEL, TC_TRSV, TC_GRAM, TC_TRSM, TC_REFINE, TC_SUPER,
+                  TC_COUNT };
 void timer_begin(int cls, hipStream_t s);
 void timer_end(int cls, hipStream_t s, double algo_flops, double algo_bytes);
 bool timer_on(int cls);
