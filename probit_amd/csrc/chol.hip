@@ -150,9 +150,14 @@ __device__ __forceinline__ void invert16(const double (*A)[TILE + 1], double (*X
 // wave inverts one diagonal sub-block, and the off-diagonal blocks of
 // X = L^-1 follow by block distance: X_rs = -X_rr sum_{k=s}^{r-1} L_rk X_ks.
 // L (lower) goes to global storage as T, X to o.dinv (FP64).
-template <typename T>
+// `published(...)` runs (all threads) once the inverse's stores are issued,
+// before L's: a caller that hands DINV to other workgroups publishes there.
+struct NoPublish {
+  __device__ void operator()() const {}
+};
+template <typename T, typename Pub = NoPublish>
 __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], double (*X)[TILE + 1], int64_t r0,
-                                     const FactorOut& o) {
+                                     const FactorOut& o, Pub published = Pub()) {
   __shared__ double dval[TILE], rinv[TILE];
   const int t = threadIdx.x;
   const int wv = t >> 6, l = t & 63;
@@ -200,16 +205,20 @@ __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], doub
     __syncthreads();
   }
   __builtin_amdgcn_s_setprio(0);
+  for (int e = t; e < TILE * TILE; e += NTHREADS) {
+    const int c = e / TILE, r = e % TILE;  // column-major walk: coalesced stores
+    // coherent: the column kernels hand DINV to other workgroups
+    st_coherent(&o.dinv[r + c * TILE], (c <= r) ? X[r][c] : 0.0);
+  }
+  published();
   if (t < TILE) {
     if (o.logdiag) o.logdiag[t] = log(dval[t]);
     if (o.diagval) o.diagval[t] = dval[t];
   }
   T* L = (T*)o.L;
   for (int e = t; e < TILE * TILE; e += NTHREADS) {
-    const int c = e / TILE, r = e % TILE;  // column-major walk: coalesced stores
+    const int c = e / TILE, r = e % TILE;
     if (c <= r) L[(size_t)(r0 + r) + (size_t)(r0 + c) * o.ld] = (T)A[r][c];
-    // coherent: the column kernel hands DINV to other workgroups
-    st_coherent(&o.dinv[r + c * TILE], (c <= r) ? X[r][c] : 0.0);
   }
 }
 
@@ -561,7 +570,10 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
 struct NoHook {
   __device__ void operator()(int) const {}
 };
-template <typename T, int BF16, typename OT, bool COH = false, typename Hook = NoHook>
+// SAME: the diagonal tile (ri == rj), whose two operand rows are the same:
+// fetched and staged once.
+template <typename T, int BF16, typename OT, bool SAME = false, bool COH = false,
+          typename Hook = NoHook>
 __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int kc0, int kc1, int64_t ri,
                                    int64_t rj, char* arena, OT* out, int opitch, bool kmajor,
                                    Hook hook = Hook()) {
@@ -571,7 +583,7 @@ __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int 
   typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
   typedef typename Acc<T>::type acc_t;
   LT* sa = (LT*)arena;
-  LT* sb = sa + TILE * PITCH;
+  LT* sb = SAME ? sa : sa + TILE * PITCH;
   const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
   acc_t acc[2][2];
   zero_acc<T>(acc);
@@ -585,10 +597,10 @@ __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int 
         const int k = operand_k<MM>(kb, q);
         if constexpr (COH) {
           ra[q] = ld_coh(&A[(size_t)(ri + m) + (size_t)(c0 + k) * ld]);
-          rb[q] = ld_coh(&A[(size_t)(rj + m) + (size_t)(c0 + k) * ld]);
+          if constexpr (!SAME) rb[q] = ld_coh(&A[(size_t)(rj + m) + (size_t)(c0 + k) * ld]);
         } else {
           ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
-          rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
+          if constexpr (!SAME) rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
         }
       }
     };
@@ -597,17 +609,17 @@ __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int 
       if (c > kc0) __syncthreads();
       if constexpr (MM) {
         stash_f32_mmajor(sa, PITCH, ra, m, kb);
-        stash_f32_mmajor(sb, PITCH, rb, m, kb);
+        if constexpr (!SAME) stash_f32_mmajor(sb, PITCH, rb, m, kb);
       } else {
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
           const int k = kb + 4 * q;
           if constexpr (BF16) {
             sa[m * PITCH + k] = f2bf((float)ra[q]);
-            sb[m * PITCH + k] = f2bf((float)rb[q]);
+            if constexpr (!SAME) sb[m * PITCH + k] = f2bf((float)rb[q]);
           } else {
             sa[k * PITCH + m] = ra[q];
-            sb[k * PITCH + m] = rb[q];
+            if constexpr (!SAME) sb[k * PITCH + m] = rb[q];
           }
         }
       }
@@ -662,7 +674,7 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
   const int64_t c0 = (int64_t)j * TILE;
   if (t < nd) {
     const Prob& P = *probs[t];
-    update_tile_to_lds<T, BF16, double>((const T*)P.L, P.Np, s0, j, c0, c0, (char*)arena,
+    update_tile_to_lds<T, BF16, double, true>((const T*)P.L, P.Np, s0, j, c0, c0, (char*)arena,
                                         &arena[0][0][0], TILE + 1, false);
     FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
                 P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
@@ -828,18 +840,26 @@ __global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, Step
     if (c == j - 1) wait_tiles(P, nb, s0, i, j, c, c + 1, epoch);
   };
   if (i == j) {
-    update_tile_to_lds<T, BF16, double, true>((const T*)P.L, P.Np, s0, j, c0, c0, (char*)arena,
-                                              &arena[0][0][0], TILE + 1, false, last_col);
+    update_tile_to_lds<T, BF16, double, true, true>((const T*)P.L, P.Np, s0, j, c0, c0,
+                                                    (char*)arena, &arena[0][0][0], TILE + 1,
+                                                    false, last_col);
     FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
                 P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
                 P.info};
-    factor_block_blocked<T>(arena[0], arena[1], c0, o);
+    // block j's rows wait only for DINV_j: publish before L_jj is stored
+    factor_block_blocked<T>(arena[0], arena[1], c0, o, [&]() {
+      publish_wait();
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_store(tile_flag(P, nb, s0, j, j), epoch, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    });
   } else {
     constexpr int PP = KP<T>::value;
     T* ck = (T*)&arena[0][0][0];
     T* bk = ck + TILE * PP;
-    update_tile_to_lds<T, BF16, T, true>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena, ck, PP,
-                                         true, last_col);
+    update_tile_to_lds<T, BF16, T, false, true>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena,
+                                                ck, PP, true, last_col);
     wait_tiles(P, nb, s0, j, j, j, j + 1, epoch);  // block j factored, DINV_j published
     const double* Dj = P.dinv + (size_t)j * TILE * TILE;
     {
