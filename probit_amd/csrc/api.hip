@@ -824,13 +824,11 @@ struct GraphEntry {
   int fit_seen;            // the last fit that used the key
   int uses;                // fits that used the key
   hipGraphExec_t exec;     // captured once the key recurs in a later fit
-  uint64_t stamp;
 };
 struct GraphCache {
   std::mutex mu;
   std::vector<GraphEntry> e;
   hipStream_t capture[64] = {};
-  uint64_t clock = 0;
   int fit_counter = 0;
 };
 static GraphCache& graph_cache() {
@@ -862,17 +860,12 @@ static hipGraphExec_t newton_graph(const GraphKey& k, int fit, const Batch& b, i
   for (GraphEntry& g : gc.e)
     if (g.key == k) ent = &g;
   if (!ent) {
-    if (gc.e.size() >= GRAPH_CACHE_MAX) {  // evict the least recently used
-      size_t lru = 0;
-      for (size_t i = 1; i < gc.e.size(); ++i)
-        if (gc.e[i].stamp < gc.e[lru].stamp) lru = i;
-      if (gc.e[lru].exec) (void)hipGraphExecDestroy(gc.e[lru].exec);
-      gc.e.erase(gc.e.begin() + lru);
-    }
-    gc.e.push_back(GraphEntry{k, fit, 1, nullptr, ++gc.clock});
+    // a full cache takes no new keys (their iterations launch directly): a
+    // graph handed out may still be in use on another thread, so none is
+    // ever destroyed before the process ends
+    if (gc.e.size() < GRAPH_CACHE_MAX) gc.e.push_back(GraphEntry{k, fit, 1, nullptr});
     return nullptr;
   }
-  ent->stamp = ++gc.clock;
   if (ent->fit_seen != fit) {
     ent->fit_seen = fit;
     ++ent->uses;

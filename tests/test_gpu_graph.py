@@ -122,3 +122,28 @@ def test_no_graph_flag_launches_directly():
     s1 = _stats(lib)
     assert s1[0] == s0[0] and s1[1] == s0[1]
     assert s1[2] - s0[2] == 3 * res.iterations
+
+
+def test_graph_cache_follows_the_schedule_table():
+    """A schedule-table change gives the iteration a new launch structure: the
+    graph captured under the old entry is not replayed for it (the cache is
+    keyed by the entry), and the fit equals a direct fit under the new entry
+    bit for bit."""
+    import torch
+
+    from probit_amd import _lib, autotune
+    lib = _lib.lib()
+    N, prec = 900, 1
+    ws = _lib.workspace(lib.pgp_workspace_size(N, 2, prec), "cuda")
+    direct_ws = _lib.workspace(lib.pgp_workspace_size(N, 2, prec), "cuda")
+    X, y, lp = _problem(97, N, "probit")
+    saved = autotune.get("fp32", 0)
+    try:
+        for _ in range(3):  # captured from the second fit on
+            _fit(lib, _lib, torch, X, y, lp, "probit", prec, ws, 0)
+        autotune.set_entry("fp32", 0, dict(saved, superstep=4 if saved["superstep"] != 4 else 6))
+        ref = _fit(lib, _lib, torch, X, y, lp, "probit", prec, direct_ws, _lib.FIT_NO_GRAPH)
+        for rep in range(3):
+            _same(_fit(lib, _lib, torch, X, y, lp, "probit", prec, ws, 0), ref, ("new entry", rep))
+    finally:
+        autotune.set_entry("fp32", 0, saved)
