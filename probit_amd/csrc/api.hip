@@ -823,19 +823,21 @@ struct GraphEntry {
   GraphKey key;
   int fit_seen;            // the last fit that used the key
   int uses;                // fits that used the key
+  int busy;                // iterations holding the graph (handed out, not yet synchronised)
   hipGraphExec_t exec;     // captured once the key recurs in a later fit
 };
+constexpr size_t GRAPH_CACHE_MAX = 64;
 struct GraphCache {
   std::mutex mu;
-  std::vector<GraphEntry> e;
+  std::vector<GraphEntry> e;  // reserved to GRAPH_CACHE_MAX: entries never move
   hipStream_t capture[64] = {};
   int fit_counter = 0;
+  GraphCache() { e.reserve(GRAPH_CACHE_MAX); }
 };
 static GraphCache& graph_cache() {
   static GraphCache c;
   return c;
 }
-constexpr size_t GRAPH_CACHE_MAX = 64;
 static std::atomic<int64_t> g_graphs_captured{0}, g_graph_launches{0}, g_direct_iters{0};
 
 
@@ -850,44 +852,78 @@ static bool graphs_enabled(int flags) {
   return true;
 }
 
+// One iteration's hold on a cache entry: the entry is not evicted while an
+// iteration holds it.  release() follows the synchronisation of the stream
+// the graph was launched on; a hold dropped without it (an error return)
+// synchronises that stream first, so an evicted graph has always finished.
+struct GraphHold {
+  GraphEntry* ent = nullptr;
+  hipStream_t s = nullptr;
+  void release() {
+    if (!ent) return;
+    std::lock_guard<std::mutex> lock(graph_cache().mu);
+    --ent->busy;
+    ent = nullptr;
+  }
+  ~GraphHold() {
+    if (ent) (void)hipStreamSynchronize(s);
+    release();
+  }
+};
+
 // The graph of key k for fit `fit` (captured now if the key was used by an
-// earlier fit), or nullptr: run the launches directly.
+// earlier fit), or nullptr: run the launches directly.  A returned graph is
+// held by `hold` until the caller has synchronised stream s.  A full cache
+// makes room by evicting the entry least recently used by a fit among those
+// no iteration holds (keys of workspaces since freed age out this way).
 static hipGraphExec_t newton_graph(const GraphKey& k, int fit, const Batch& b, int prec,
-                                   int refine, double* gather) {
+                                   int refine, double* gather, hipStream_t s,
+                                   GraphHold& hold) {
   GraphCache& gc = graph_cache();
   std::lock_guard<std::mutex> lock(gc.mu);
   GraphEntry* ent = nullptr;
   for (GraphEntry& g : gc.e)
     if (g.key == k) ent = &g;
   if (!ent) {
-    // a full cache takes no new keys (their iterations launch directly): a
-    // graph handed out may still be in use on another thread, so none is
-    // ever destroyed before the process ends
-    if (gc.e.size() < GRAPH_CACHE_MAX) gc.e.push_back(GraphEntry{k, fit, 1, nullptr});
+    if (gc.e.size() < GRAPH_CACHE_MAX) {
+      gc.e.push_back(GraphEntry{k, fit, 1, 0, nullptr});
+      return nullptr;
+    }
+    GraphEntry* old = nullptr;
+    for (GraphEntry& g : gc.e)
+      if (g.busy == 0 && (!old || g.fit_seen < old->fit_seen)) old = &g;
+    if (!old) return nullptr;  // every entry held: launch directly
+    if (old->exec) (void)hipGraphExecDestroy(old->exec);
+    *old = GraphEntry{k, fit, 1, 0, nullptr};
     return nullptr;
   }
   if (ent->fit_seen != fit) {
     ent->fit_seen = fit;
     ++ent->uses;
   }
-  if (ent->exec || ent->uses < 2) return ent->exec;
-  if (k.dev < 0 || k.dev >= 64) return nullptr;
-  if (!gc.capture[k.dev] &&
-      hipStreamCreateWithFlags(&gc.capture[k.dev], hipStreamNonBlocking) != hipSuccess)
-    return nullptr;
-  hipStream_t cs = gc.capture[k.dev];
-  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
-  newton_iter(b, k.first ? nullptr : &b, k.refactor ? &b : nullptr, nullptr,
-              k.refactor ? nullptr : &b, prec, refine, gather, cs);
-  hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(cs, &g);
-  hipGraphExec_t exec = nullptr;
-  if (e == hipSuccess && g && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess) {
-    ent->exec = exec;
-    ++g_graphs_captured;
+  if (!ent->exec && ent->uses >= 2 && k.dev >= 0 && k.dev < 64) {
+    if (!gc.capture[k.dev] &&
+        hipStreamCreateWithFlags(&gc.capture[k.dev], hipStreamNonBlocking) != hipSuccess)
+      return nullptr;
+    hipStream_t cs = gc.capture[k.dev];
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    newton_iter(b, k.first ? nullptr : &b, k.refactor ? &b : nullptr, nullptr,
+                k.refactor ? nullptr : &b, prec, refine, gather, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess && g && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess) {
+      ent->exec = exec;
+      ++g_graphs_captured;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();  // a failed capture leaves the launches to run directly
   }
-  if (g) (void)hipGraphDestroy(g);
-  (void)hipGetLastError();  // a failed capture leaves the launches to run directly
+  if (ent->exec) {
+    ++ent->busy;
+    hold.ent = ent;
+    hold.s = s;
+  }
   return ent->exec;
 }
 
@@ -943,10 +979,11 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
     const bool chord = chord_step(cur, gauss, refactor);
     force_factor = false;
     hipGraphExec_t gexec = nullptr;
+    GraphHold hold;
     if (use_graphs)
       gexec = newton_graph(GraphKey{dev, ws, (int)p.Np, cur, refactor ? 1 : 0, refine,
                                     it == 0 ? 1 : 0, tune_get(cur, tune_class_of((int)p.Np))},
-                           fit_id, b, cur, refine, gather);
+                           fit_id, b, cur, refine, gather, s, hold);
     if (gexec) {
       HIP_OK(hipGraphLaunch(gexec, s));
       ++g_graph_launches;
@@ -962,6 +999,7 @@ static int laplace_fit_ws(const pgp_problem_t* problem, const pgp_kernel_t* kern
     HIP_OK(hipMemcpyAsync(&h2, gather, sizeof h2, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&inf0, ws + OFF_INFO, sizeof inf0, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    hold.release();
     const double st = sqrt(h2);
     if (cur == PGP_PREC_BF16 && ((inf0 & PGP_INFO_NOT_PD) || !(st == st))) {
       // the step was not taken (OP_ZUPD); redo the iteration with FP32 updates

@@ -147,3 +147,31 @@ def test_graph_cache_follows_the_schedule_table():
             _same(_fit(lib, _lib, torch, X, y, lp, "probit", prec, ws, 0), ref, ("new entry", rep))
     finally:
         autotune.set_entry("fp32", 0, saved)
+
+
+def test_full_cache_evicts_the_least_recently_used_keys():
+    """More keys than the cache holds (40 live workspaces, two or three keys
+    each): a new workspace's iterations still get captured and replayed --
+    the oldest keys make room -- and equal direct launches bit for bit; the
+    first workspace, whose keys were evicted, fits correctly again."""
+    import torch
+
+    from probit_amd import _lib
+    lib = _lib.lib()
+    N, prec = 128, 1
+    size = lib.pgp_workspace_size(N, 2, prec)
+    X, y, lp = _problem(98, N, "probit")
+    ref = _fit(lib, _lib, torch, X, y, lp, "probit", prec, _lib.workspace(size, "cuda"),
+               _lib.FIT_NO_GRAPH)
+    wss = [_lib.workspace(size, "cuda") for _ in range(40)]
+    for ws in wss:
+        for _ in range(2):
+            _same(_fit(lib, _lib, torch, X, y, lp, "probit", prec, ws, 0), ref, "filling")
+    ws = _lib.workspace(size, "cuda")
+    s0 = _stats(lib)
+    for rep in range(3):
+        _same(_fit(lib, _lib, torch, X, y, lp, "probit", prec, ws, 0), ref, ("new ws", rep))
+    s1 = _stats(lib)
+    assert s1[1] - s0[1] >= 2 * ref[1].iterations, (s0, s1)
+    for rep in range(2):
+        _same(_fit(lib, _lib, torch, X, y, lp, "probit", prec, wss[0], 0), ref, ("first ws", rep))
