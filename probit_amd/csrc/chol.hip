@@ -78,6 +78,26 @@ __device__ __forceinline__ void mma16(const double (*A)[TILE + 1], int ar, int a
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
   }
 }
+// mma16 on row-major LDS arrays of row pitch PA (A) and PB (Bm).
+template <int PA, int PB>
+__device__ __forceinline__ void mma16p(const double* A, int ar, int ac, const double* Bm, int br,
+                                       int bc, int bt, doublex4& acc) {
+  const int l = threadIdx.x & 63;
+  const int lr = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int kk = 0; kk < 16; kk += 4) {
+    const double a = A[(ar + lr) * PA + ac + kk + lk];
+    const double b = bt ? Bm[(br + lr) * PB + bc + kk + lk] : Bm[(br + kk + lk) * PB + bc + lr];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+}
+
+// LDS of the diagonal-block factor: the block A ([64][65] doubles) and the
+// four 16x16 diagonal blocks of its inverse ([4][16][17]); the inverse's
+// off-diagonal blocks live, transposed, in A's upper triangle.
+constexpr int XD_PITCH = 17;
+constexpr int XD_BLK = 16 * XD_PITCH;
+constexpr int FACTOR_LDS = (TILE * (TILE + 1) + 4 * XD_BLK) * 8;
 
 // Factor the 16-column panel A(o.., o..o+15) of a 64x64 block in place: one
 // wave, lane l holds row o + l (16 values in registers; lanes past the
@@ -123,8 +143,8 @@ __device__ __forceinline__ void panel16(double (*A)[TILE + 1], int o, double* dv
 // X(o.., o..) = inverse of the factored 16x16 diagonal sub-block at (o, o)
 // (zero above the diagonal).  One wave; lane c runs forward substitution on
 // column c of I, reading L from LDS (one address per step: a broadcast).
-__device__ __forceinline__ void invert16(const double (*A)[TILE + 1], double (*X)[TILE + 1],
-                                         int o, const double* rinv) {
+__device__ __forceinline__ void invert16(const double (*A)[TILE + 1], double* Xb, int o,
+                                         const double* rinv) {
   const int c = threadIdx.x & 15;
   double x[16];
 #pragma unroll
@@ -140,7 +160,7 @@ __device__ __forceinline__ void invert16(const double (*A)[TILE + 1], double (*X
   }
   if ((threadIdx.x & 63) < 16) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) X[o + r][o + c] = (r >= c) ? x[r] : 0.0;
+    for (int r = 0; r < 16; ++r) Xb[r * XD_PITCH + c] = (r >= c) ? x[r] : 0.0;
   }
 }
 
@@ -149,6 +169,9 @@ __device__ __forceinline__ void invert16(const double (*A)[TILE + 1], double (*X
 // products apply it to the trailing blocks, A_rc -= L_rs L_cs^T.  Then each
 // wave inverts one diagonal sub-block, and the off-diagonal blocks of
 // X = L^-1 follow by block distance: X_rs = -X_rr sum_{k=s}^{r-1} L_rk X_ks.
+// X's diagonal blocks go to Xd ([4][16][17]); X_rs (r > s) is kept
+// transposed in A's upper block (s, r), free once the factor is done, after
+// serving as the scratch of T_rs = sum_k L_rk X_ks.
 // L (lower) goes to global storage as T, X to o.dinv (FP64).
 // `published(...)` runs (all threads) once the inverse's stores are issued,
 // before L's: a caller that hands DINV to other workgroups publishes there.
@@ -156,7 +179,7 @@ struct NoPublish {
   __device__ void operator()() const {}
 };
 template <typename T, typename Pub = NoPublish>
-__device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], double (*X)[TILE + 1], int64_t r0,
+__device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], double* Xd, int64_t r0,
                                      const FactorOut& o, Pub published = Pub()) {
   __shared__ double dval[TILE], rinv[TILE];
   const int t = threadIdx.x;
@@ -182,15 +205,20 @@ __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], doub
     }
     __syncthreads();
   }
-  invert16(A, X, 16 * wv, rinv);
+  invert16(A, Xd + wv * XD_BLK, 16 * wv, rinv);
   __syncthreads();
   // off-diagonal blocks of X by distance d; T goes to A's upper block (s, r)
+  // and is replaced there by X_rs transposed: X_ks (k > s) is read back as
+  // B(kk, j) = A(16 s + j, 16 k + kk)
+  const double* Af = &A[0][0];
   for (int d = 1; d < 4; ++d) {
     const int nblk = 4 - d;
     if (wv < nblk) {
       const int r = d + wv, s = wv;
       doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-      for (int k = s; k < r; ++k) mma16(A, 16 * r, 16 * k, X, 16 * k, 16 * s, 0, acc);
+      mma16p<TILE + 1, XD_PITCH>(Af, 16 * r, 16 * s, Xd + s * XD_BLK, 0, 0, 0, acc);
+      for (int k = s + 1; k < r; ++k)
+        mma16p<TILE + 1, TILE + 1>(Af, 16 * r, 16 * k, Af, 16 * s, 16 * k, 1, acc);
 #pragma unroll
       for (int q = 0; q < 4; ++q) A[16 * s + orow + 4 * q][16 * r + ocol] = acc[q];
     }
@@ -198,17 +226,21 @@ __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], doub
     if (wv < nblk) {
       const int r = d + wv, s = wv;
       doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-      mma16(X, 16 * r, 16 * r, A, 16 * s, 16 * r, 0, acc);
+      mma16p<XD_PITCH, TILE + 1>(Xd + r * XD_BLK, 0, 0, Af, 16 * s, 16 * r, 0, acc);
+      // (the product has read all of T_rs before its result exists)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) X[16 * r + orow + 4 * q][16 * s + ocol] = -acc[q];
+      for (int q = 0; q < 4; ++q) A[16 * s + ocol][16 * r + orow + 4 * q] = -acc[q];
     }
     __syncthreads();
   }
   __builtin_amdgcn_s_setprio(0);
   for (int e = t; e < TILE * TILE; e += NTHREADS) {
     const int c = e / TILE, r = e % TILE;  // column-major walk: coalesced stores
+    double v = 0.0;
+    if (c <= r)
+      v = (r >> 4) == (c >> 4) ? Xd[(r >> 4) * XD_BLK + (r & 15) * XD_PITCH + (c & 15)] : A[c][r];
     // coherent: the column kernels hand DINV to other workgroups
-    st_coherent(&o.dinv[r + c * TILE], (c <= r) ? X[r][c] : 0.0);
+    st_coherent(&o.dinv[r + c * TILE], v);
   }
   published();
   if (t < TILE) {
@@ -225,14 +257,15 @@ __device__ __forceinline__ void factor_block_blocked(double (*A)[TILE + 1], doub
 // ---- F: first diagonal block -------------------------------------------------
 template <typename T>
 __device__ void factor_first(const T* A, int64_t ld, const FactorOut& o) {
-  __shared__ double blk[2][TILE][TILE + 1];
+  __shared__ double blk[TILE][TILE + 1];
+  __shared__ double xd[4 * XD_BLK];
   const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
   for (int q = 0; q < 16; ++q) {
     const int c = kb + 4 * q;
-    blk[0][m][c] = (double)A[(size_t)m + (size_t)c * ld];
+    blk[m][c] = (double)A[(size_t)m + (size_t)c * ld];
   }
   __syncthreads();
-  factor_block_blocked<T>(blk[0], blk[1], 0, o);
+  factor_block_blocked<T>(blk, xd, 0, o);
 }
 
 template <typename T>
@@ -291,7 +324,7 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, 
   // two FP64 blocks of the diagonal factorisation
   constexpr int PP = KP<T>::value;
   constexpr int OPER = BF16 ? 2 * TILE * 72 * 2 : 2 * TILE * PP * (int)sizeof(T);
-  constexpr int BLK = FUSE ? 2 * TILE * (TILE + 1) * 8 : 0;
+  constexpr int BLK = FUSE ? FACTOR_LDS : 0;
   __shared__ __attribute__((aligned(16))) char arena[OPER > BLK ? OPER : BLK];
   typedef typename Acc<T>::type acc_t;
   acc_t acc[2][2];
@@ -341,7 +374,7 @@ __device__ void syrk_tile(T* A, int64_t ld, int64_t c0, int64_t ri, int64_t rj, 
   if constexpr (FUSE) {
     if (factor) {
       __syncthreads();
-      factor_block_blocked<T>(blk, blk + TILE, ri, o);
+      factor_block_blocked<T>(blk, (double*)(blk + TILE), ri, o);
     }
   }
 }
@@ -400,7 +433,7 @@ __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int 
   constexpr int PITCH = BF16 ? 72 : MM ? 68 : KP<T>::value;
   typedef typename std::conditional<BF16 != 0, unsigned short, T>::type LT;
   constexpr int OPER = 2 * TILE * PITCH * (int)sizeof(LT);
-  constexpr int BLK = FUSE ? 2 * TILE * (TILE + 1) * 8 : 0;
+  constexpr int BLK = FUSE ? FACTOR_LDS : 0;
   __shared__ __attribute__((aligned(16))) char arena[OPER > BLK ? OPER : BLK];
   LT* sa = (LT*)arena;
   LT* sb = sa + TILE * PITCH;
@@ -487,7 +520,7 @@ __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int 
   if constexpr (FUSE) {
     if (factor) {
       __syncthreads();
-      factor_block_blocked<T>(blk, blk + TILE, ri, o);
+      factor_block_blocked<T>(blk, (double*)(blk + TILE), ri, o);
     }
   }
 }
@@ -652,6 +685,19 @@ __device__ __forceinline__ void update_tile_to_lds(const T* A_, int64_t ld, int 
   __syncthreads();
 }
 
+// LDS of the column kernels, used in turn by a tile update's operands, its
+// FP64 result with the diagonal factor's inverse blocks (FACTOR_LDS), or a
+// row tile's panel-product operands.
+template <typename T, int BF16>
+constexpr int col_arena_bytes() {
+  constexpr int PP = KP<T>::value;
+  constexpr int PITCH = BF16 ? 72 : MmajorF32<T, BF16>::value ? 68 : PP;
+  constexpr int oper = 2 * TILE * PITCH * (BF16 ? 2 : (int)sizeof(T));
+  constexpr int prod = 2 * TILE * PP * (int)sizeof(T);
+  constexpr int a = oper > prod ? oper : prod;
+  return a > FACTOR_LDS ? a : FACTOR_LDS;
+}
+
 // Column j of every member in one launch.  Tickets [0, nd) are the diagonal
 // workgroups of members 0..nd-1 (members are sorted by size, so these are
 // the members with more than j block columns); the rest are row workgroups
@@ -664,7 +710,9 @@ template <typename T, int BF16>
 __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched rows, int nd,
                                                      uint32_t* ticket, uint32_t epoch, int j,
                                                      int s0) {
-  __shared__ __attribute__((aligned(16))) double arena[2][TILE][TILE + 1];
+  __shared__ __attribute__((aligned(16))) char arena_b[col_arena_bytes<T, BF16>()];
+  double (*arena)[TILE + 1] = (double (*)[TILE + 1])arena_b;
+  double* xd = (double*)arena_b + TILE * (TILE + 1);
   __shared__ int s_val;
   const int t = take_ticket(ticket, &s_val);
   if ((unsigned)t >= gridDim.x) {  // counter not zero at launch: nothing to take
@@ -675,11 +723,11 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
   if (t < nd) {
     const Prob& P = *probs[t];
     update_tile_to_lds<T, BF16, double, true>((const T*)P.L, P.Np, s0, j, c0, c0, (char*)arena,
-                                        &arena[0][0][0], TILE + 1, false);
+                                        &arena[0][0], TILE + 1, false);
     FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
                 P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
                 P.info};
-    factor_block_blocked<T>(arena[0], arena[1], c0, o);
+    factor_block_blocked<T>(arena, xd, c0, o);
     // every wave's DINV stores complete, then publish
     publish_wait();
     __syncthreads();
@@ -693,7 +741,7 @@ __global__ __launch_bounds__(NTHREADS) void k_column(Prob* const* probs, Sched r
   const int64_t ri = c0 + (int64_t)TILE * (local + 1);
   // the panel product runs in the storage type (as panel_rows)
   constexpr int PP = KP<T>::value;
-  T* ck = (T*)&arena[0][0][0];
+  T* ck = (T*)&arena[0][0];
   T* bk = ck + TILE * PP;
   update_tile_to_lds<T, BF16, T>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena, ck, PP, true);
   if (threadIdx.x == 0) wait_progress(P.progress, epoch, 1, P.info);
@@ -795,7 +843,9 @@ __device__ __forceinline__ void wait_tiles(const Prob& P, int nb, int s0, int i,
 template <typename T, int BF16>
 __global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, StepSched ss,
                                                         uint32_t* ticket, uint32_t epoch) {
-  __shared__ __attribute__((aligned(16))) double arena[2][TILE][TILE + 1];
+  __shared__ __attribute__((aligned(16))) char arena_b[col_arena_bytes<T, BF16>()];
+  double (*arena)[TILE + 1] = (double (*)[TILE + 1])arena_b;
+  double* xd = (double*)arena_b + TILE * (TILE + 1);
   __shared__ int s_val;
   const int t = take_ticket(ticket, &s_val);
   if ((unsigned)t >= gridDim.x) {  // counter not zero at launch: nothing to take
@@ -841,13 +891,13 @@ __global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, Step
   };
   if (i == j) {
     update_tile_to_lds<T, BF16, double, true, true>((const T*)P.L, P.Np, s0, j, c0, c0,
-                                                    (char*)arena, &arena[0][0][0], TILE + 1,
+                                                    (char*)arena, &arena[0][0], TILE + 1,
                                                     false, last_col);
     FactorOut o{P.L, P.Np, P.dinv + (size_t)j * TILE * TILE,
                 P.logdiag ? P.logdiag + c0 : nullptr, P.diagval ? P.diagval + c0 : nullptr,
                 P.info};
     // block j's rows wait only for DINV_j: publish before L_jj is stored
-    factor_block_blocked<T>(arena[0], arena[1], c0, o, [&]() {
+    factor_block_blocked<T>(arena, xd, c0, o, [&]() {
       publish_wait();
       __syncthreads();
       if (threadIdx.x == 0)
@@ -856,7 +906,7 @@ __global__ __launch_bounds__(NTHREADS) void k_superstep(Prob* const* probs, Step
     });
   } else {
     constexpr int PP = KP<T>::value;
-    T* ck = (T*)&arena[0][0][0];
+    T* ck = (T*)&arena[0][0];
     T* bk = ck + TILE * PP;
     update_tile_to_lds<T, BF16, T, false, true>((const T*)P.L, P.Np, s0, j, ri, c0, (char*)arena,
                                                 ck, PP, true, last_col);
@@ -914,17 +964,18 @@ __global__ __launch_bounds__(NTHREADS) void k_factor_diag(Prob* const* probs, Sc
   const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
   const int64_t c0 = (int64_t)k * TILE;
   if (c0 >= P.Np) return;
-  __shared__ double blk[2][TILE][TILE + 1];
+  __shared__ double blk[TILE][TILE + 1];
+  __shared__ double xd[4 * XD_BLK];
   const T* A = (const T*)P.L;
   const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
   for (int q = 0; q < 16; ++q) {
     const int c = kb + 4 * q;
-    blk[0][m][c] = (double)A[(size_t)(c0 + m) + (size_t)(c0 + c) * P.Np];
+    blk[m][c] = (double)A[(size_t)(c0 + m) + (size_t)(c0 + c) * P.Np];
   }
   __syncthreads();
   FactorOut o{P.L, P.Np, P.dinv + (size_t)k * TILE * TILE, P.logdiag ? P.logdiag + c0 : nullptr,
               P.diagval ? P.diagval + c0 : nullptr, P.info};
-  factor_block_blocked<T>(blk[0], blk[1], c0, o);
+  factor_block_blocked<T>(blk, xd, c0, o);
 }
 
 // ---- raw (kernel-level ABI) variants ----------------------------------------
