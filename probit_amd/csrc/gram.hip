@@ -421,43 +421,45 @@ __global__ __launch_bounds__(NTHREADS) void k_pred_refine(Prob* const* probs, Sc
   if (bx >= nrt * nct) return;
   const int ti = bx % nrt, tj = bx / nrt;
   constexpr int PA = TILE + 1;
-  __shared__ double sa[TILE * PA], sb[TILE * PA];
+  constexpr int KC = 32;  // k rows staged per step (half a tile: LDS and
+                          // prefetch registers for more resident workgroups)
+  __shared__ double sa[KC * PA], sb[KC * PA];
   __shared__ double red[2][TILE];
   __shared__ KShared ks;
-  static_assert(TILE * PGP_MAX_DIM <= TILE * PA, "input rows fit the operand buffers");
+  static_assert(TILE * PGP_MAX_DIM <= KC * PA, "input rows fit the operand buffers");
   const int D = P.D, M = P.M;
   const size_t np = (size_t)P.Np;
   const double* K = P.K;
   const T* Z = (const T*)P.S;
   const double* Ws = vslot(P, V_S);
-  const int kl = threadIdx.x & 63, cg = threadIdx.x >> 6;
+  const int t = threadIdx.x;
+  const int am = t & 63, ag = t >> 6;  // A: row m, k = ag + 4q
+  const int bk = t & 31, bg = t >> 5;  // B: row k, n = bg + 8q
   doublex4 acc[2][2];
   zero_acc<double>(acc);
-  // ra[q] = K(64 ti + kl, k0 + cg + 4q); rb[q] = c U(k0 + kl, 64 tj + cg + 4q),
+  // ra[q] = K(64 ti + am, k0 + ag + 4q); rb[q] = c U(k0 + bk, 64 tj + bg + 8q),
   // c = 2 left of the diagonal tile, 1 on it
-  double ra[16], rb[16];
+  double ra[8], rb[8];
   const int kend = ti * TILE;  // the diagonal tile's first column
   auto fetch = [&](int k0) {
-    const double w = (k0 < kend ? 2.0 : 1.0) * Ws[k0 + kl];
+    const double w = (k0 < kend ? 2.0 : 1.0) * Ws[k0 + bk];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = cg + 4 * q;
-      ra[q] = K[(size_t)(ti * TILE + kl) + (size_t)(k0 + c) * np];
-      rb[q] = w * (double)Z[(size_t)(k0 + kl) + (size_t)(tj * TILE + c) * np];
+    for (int q = 0; q < 8; ++q) {
+      ra[q] = K[(size_t)(ti * TILE + am) + (size_t)(k0 + ag + 4 * q) * np];
+      rb[q] = w * (double)Z[(size_t)(k0 + bk) + (size_t)(tj * TILE + bg + 8 * q) * np];
     }
   };
   fetch(0);
-  for (int k0 = 0; k0 <= kend; k0 += TILE) {
+  for (int k0 = 0; k0 < kend + TILE; k0 += KC) {
     __syncthreads();  // the previous chunk's products are done with LDS
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = cg + 4 * q;
-      sa[c * PA + kl] = ra[q];  // A(m = kl, k = c), k-major
-      sb[kl * PA + c] = rb[q];  // B(k = kl, n = c), k-major
+    for (int q = 0; q < 8; ++q) {
+      sa[(ag + 4 * q) * PA + am] = ra[q];  // A(m, k), k-major
+      sb[bk * PA + bg + 8 * q] = rb[q];    // B(k, n), k-major
     }
     __syncthreads();
-    if (k0 + TILE <= kend) fetch(k0 + TILE);
-    mma64_kmajor<double>(sa, PA, sb, PA, acc);
+    if (k0 + KC < kend + TILE) fetch(k0 + KC);
+    mma64_kmajor_k32<double>(sa, PA, sb, PA, acc);
   }
   __syncthreads();  // operands are dead: the input rows take their place
   double* xs = sa;

@@ -88,6 +88,28 @@ __device__ __forceinline__ void mma64_kmajor(const T* As, int pa, const T* Bs, i
   }
 }
 
+// mma64_kmajor over K = 32 (k-major operands of 32 rows): the first or the
+// second half of mma64_kmajor's k steps, in the same order.
+template <typename T>
+__device__ __forceinline__ void mma64_kmajor_k32(const T* As, int pa, const T* Bs, int pb,
+                                                 typename Acc<T>::type (&acc)[2][2]) {
+  const int l = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int wr = 32 * (w >> 1), wc = 32 * (w & 1);
+  const int lr = l & 15, lk = l >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < 32; kk += 4) {
+    const T a0 = As[(kk + lk) * pa + wr + lr];
+    const T a1 = As[(kk + lk) * pa + wr + 16 + lr];
+    const T b0 = Bs[(kk + lk) * pb + wc + lr];
+    const T b1 = Bs[(kk + lk) * pb + wc + 16 + lr];
+    acc[0][0] = mfma16(a0, b0, acc[0][0]);
+    acc[0][1] = mfma16(a0, b1, acc[0][1]);
+    acc[1][0] = mfma16(a1, b0, acc[1][0]);
+    acc[1][1] = mfma16(a1, b1, acc[1][1]);
+  }
+}
+
 // acc += A B^T over K = 64 for float, m-major LDS operands As[m * pa + k],
 // Bs[n * pb + k] (pa, pb multiples of 4; with pa = pb = 4 mod 32 the 16-byte
 // reads of 8 consecutive lanes cover the 32 banks once; the counters still
