@@ -66,7 +66,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--requests-per-step", type=int, default=512)
     p.add_argument("--precision", default="fp32", choices=["bf16", "fp32", "fp64"])
-    p.add_argument("--capacity", type=int, default=1024,
+    p.add_argument("--capacity", type=int, default=2048,
                    help="request slots of the serving engine (requests in flight)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-sized device buffer 3 times and exit (the known-size "
@@ -548,6 +548,8 @@ def main():
             "p99_latency_ms": round(1e3 * float(np.percentile(lat_open, 99)), 3),
             "latency_from": "arrival",
         },
+        "hbm_in_flight": {"peak_gb": round(eng.peak_used / 1e9, 2),
+                          "budget_gb": round(eng.budget / 1e9, 2)},
         "closed_loop": {
             "p50_latency_ms": round(1e3 * float(np.percentile(lat_closed, 50)), 3),
             "p99_latency_ms": round(1e3 * float(np.percentile(lat_closed, 99)), 3),

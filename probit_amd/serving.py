@@ -51,6 +51,7 @@ class ServingEngine:
             hbm_budget_bytes = int(free * 0.8)
         self.budget = int(hbm_budget_bytes)
         self.used = 0
+        self.peak_used = 0  # the most bytes of admitted requests held at once
         opts = _lib.FitOptionsT()
         opts.tolerance = tolerance
         opts.maxiter = maxiter
@@ -164,6 +165,7 @@ class ServingEngine:
         s.t_submit = time.perf_counter()
         self._slots[int(tag)] = s
         self.used += s.nbytes
+        self.peak_used = max(self.peak_used, self.used)
         return True
 
     @staticmethod
