@@ -532,7 +532,7 @@ def main():
                         "EQ / Matern12, 256 test points; approximate_posterior + predict",
             "requests_per_step": R,
             "capacity": args.capacity,
-            "lane_np": os.environ.get("PGP_LANE_NP", "1024"),
+            "lane_np": os.environ.get("PGP_LANE_NP", "1024,1536,2560"),
             "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
             "min_n": args.min_n,
             "precision": args.precision,

@@ -450,10 +450,10 @@ int pgp_batcher_submit(pgp_batcher_t* batcher, pgp_batch_item_t* item,
 int pgp_batcher_step(pgp_batcher_t* batcher, int64_t* done_tags,
                      int32_t max_done, int32_t* n_done);
 /* Asynchronous form of pgp_batcher_step, so that the host can prepare and
- * submit requests while the GPU works.  Items are split by size into two
- * lanes (padded N above 1024, or PGP_LANE_NP in the environment, takes the
- * second), each with its own streams and step cadence, so that the small
- * items run several steps while the large ones run one.  launch() admits
+ * submit requests while the GPU works.  Items are split by padded size into
+ * four lanes (bounds 1024, 1536 and 2560, or PGP_LANE_NP in the
+ * environment), each with its own streams and step cadence, so that the
+ * small items run several steps while the large ones run one.  launch() admits
  * queued items and, on every lane without a step in flight, enqueues one
  * step (the finalisation and predict of the lane's items that converged in
  * its previous step, one Newton iteration for its other items), returning

@@ -1937,13 +1937,13 @@ static void lane_finish(pgp_batcher* bt, int li, int64_t* done_tags, int32_t max
 }
 
 // Size-class bounds on the padded N (PGP_LANE_NP: up to NLANES - 1
-// ascending values separated by commas, default "1024"; "0" keeps every
+// ascending values separated by commas, default "1024,1536,2560"; "0" keeps every
 // member in lane 0).  A member takes the lane of the number of bounds it
 // exceeds.
 static void lane_bounds(int* out) {
   for (int k = 0; k < NLANES - 1; ++k) out[k] = 0;
   const char* e = getenv("PGP_LANE_NP");
-  const char* p = e ? e : "1024";
+  const char* p = e ? e : "1024,1536,2560";
   for (int k = 0; k < NLANES - 1 && *p; ++k) {
     out[k] = atoi(p);
     while (*p && *p != ',') ++p;
