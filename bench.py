@@ -533,6 +533,7 @@ def main():
             "requests_per_step": R,
             "capacity": args.capacity,
             "lane_np": os.environ.get("PGP_LANE_NP", "1024,1536,2560"),
+            "lane_split_min": int(os.environ.get("PGP_LANE_SPLIT_MIN", "512")),
             "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
             "min_n": args.min_n,
             "precision": args.precision,

@@ -452,8 +452,10 @@ int pgp_batcher_step(pgp_batcher_t* batcher, int64_t* done_tags,
 /* Asynchronous form of pgp_batcher_step, so that the host can prepare and
  * submit requests while the GPU works.  Items are split by padded size into
  * four lanes (bounds 1024, 1536 and 2560, or PGP_LANE_NP in the
- * environment), each with its own streams and step cadence, so that the
- * small items run several steps while the large ones run one.  launch() admits
+ * environment; the lanes past the second take items only while at least 512
+ * are in flight, PGP_LANE_SPLIT_MIN), each with its own streams and step
+ * cadence, so that the small items run several steps while the large ones
+ * run one.  launch() admits
  * queued items and, on every lane without a step in flight, enqueues one
  * step (the finalisation and predict of the lane's items that converged in
  * its previous step, one Newton iteration for its other items), returning

@@ -1628,6 +1628,7 @@ struct pgp_batcher {
   int64_t lane_steps = 0;
 
   int lane_np[pgp::NLANES - 1];  // size class bounds on Np (0: unused)
+  int split_min = 0;  // members in flight from which lanes past 1 take members
   pgp::Lane lane[pgp::NLANES];
   std::list<pgp::BItem> items;  // stable addresses: lanes hold pointers
   std::vector<pgp::BItem> pending;
@@ -1696,6 +1697,8 @@ static void admit(pgp_batcher* bt) {
     b.lane = 0;
     for (int k = 0; k < NLANES - 1; ++k)
       if (bt->lane_np[k] > 0 && b.p.Np > bt->lane_np[k]) b.lane = k + 1;
+    // a lightly loaded engine keeps every large member in lane 1
+    if (b.lane > 1 && (int)bt->items.size() < bt->split_min) b.lane = 1;
     bt->items.push_back(b);
   }
   bt->pending.erase(bt->pending.begin(), bt->pending.begin() + take);
@@ -1940,6 +1943,15 @@ static void lane_finish(pgp_batcher* bt, int li, int64_t* done_tags, int32_t max
 // ascending values separated by commas, default "1024,1536,2560"; "0" keeps every
 // member in lane 0).  A member takes the lane of the number of bounds it
 // exceeds.
+// Members in flight from which a new member may take a lane past the
+// second (PGP_LANE_SPLIT_MIN, default 512; 0: always): splitting the large
+// members over more lanes raises the throughput of a full engine but slows
+// the small members' lane while the engine is lightly loaded.
+static int lane_split_min() {
+  const char* e = getenv("PGP_LANE_SPLIT_MIN");
+  return e ? std::max(0, atoi(e)) : 512;
+}
+
 static void lane_bounds(int* out) {
   for (int k = 0; k < NLANES - 1; ++k) out[k] = 0;
   const char* e = getenv("PGP_LANE_NP");
@@ -1973,6 +1985,7 @@ int pgp_batcher_create(int32_t capacity, const pgp_fit_options_t* opts, void* ta
   b->refine = opts->refine >= 0 ? opts->refine : default_refine(opts->precision);
   b->cap = capacity;
   lane_bounds(b->lane_np);
+  b->split_min = lane_split_min();
   bool ok = true;
   b->caller = (hipStream_t)stream;
   ok = hipEventCreateWithFlags(&b->ev_caller, hipEventDisableTiming) == hipSuccess;

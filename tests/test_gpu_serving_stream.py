@@ -161,3 +161,51 @@ def test_large_lane_converted_inputs_async_equals_step():
         for name in ("mean", "var", "weight", "precision"):
             assert torch.equal(getattr(a[t], name), getattr(s[t], name)), (t, name)
         assert a[t].iterations == s[t].iterations and a[t].status == s[t].status == 0
+
+
+@pytest.mark.parametrize("split_min", ["0", "512"])
+def test_size_lanes_async_equals_step(monkeypatch, split_min):
+    """Requests in all four size lanes (padded N up to 1024, 1536, 2560 and
+    above), which a loaded engine splits the large ones over (here from the
+    first member on, PGP_LANE_SPLIT_MIN=0) and a lightly loaded one keeps in
+    the second lane (the default, 512 members in flight): launch / finish
+    gives the bits of the synchronous step, which has no lanes."""
+    import torch
+
+    from probit_amd.serving import ServingEngine
+    monkeypatch.setenv("PGP_LANE_SPLIT_MIN", split_min)
+    rng = np.random.default_rng(71)
+    probs = []
+    for k, N in enumerate((300, 1200, 2000, 3000, 800, 1500, 2600, 100)):
+        X = rng.uniform(-1, 1, (N, 2))
+        f = np.sin(3 * X.sum(1))
+        Xt = rng.uniform(-1, 1, (256, 2))
+        if k % 2:
+            cut = np.array([-np.inf, -0.3, 0.4, np.inf])
+            y = np.searchsorted(cut[1:-1], f + 0.4 * rng.normal(size=N)).astype(np.float64)
+            probs.append((X, y, "probit", 0.5, cut, Xt))
+        else:
+            probs.append((X, f + 0.2 * rng.normal(size=N), "gaussian", 0.2, None, Xt))
+
+    def run(async_):
+        eng = ServingEngine(precision="fp32", capacity=8)
+        for tag, (X, y, lik, noise, cut, Xt) in enumerate(probs):
+            assert eng.submit(tag, X, y, "EQ", 0.6, 1.1, lik, noise, cut, Xt)
+        out = {}
+        while eng.in_flight:
+            if async_:
+                if eng.launch():
+                    for c in eng.finish():
+                        out[c.tag] = c
+            else:
+                for c in eng.step():
+                    out[c.tag] = c
+        eng.close()
+        return out
+
+    a, s = run(True), run(False)
+    assert sorted(a) == sorted(s) == list(range(len(probs)))
+    for t in a:
+        for name in ("mean", "var", "weight", "precision"):
+            assert torch.equal(getattr(a[t], name), getattr(s[t], name)), (t, name)
+        assert a[t].iterations == s[t].iterations and a[t].status == s[t].status == 0
