@@ -1939,10 +1939,6 @@ static void lane_finish(pgp_batcher* bt, int li, int64_t* done_tags, int32_t max
   drop_done(bt);
 }
 
-// Size-class bounds on the padded N (PGP_LANE_NP: up to NLANES - 1
-// ascending values separated by commas, default "1024,1536,2560"; "0" keeps every
-// member in lane 0).  A member takes the lane of the number of bounds it
-// exceeds.
 // Members in flight from which a new member may take a lane past the
 // second (PGP_LANE_SPLIT_MIN, default 512; 0: always): splitting the large
 // members over more lanes raises the throughput of a full engine but slows
@@ -1952,6 +1948,10 @@ static int lane_split_min() {
   return e ? std::max(0, atoi(e)) : 512;
 }
 
+// Size-class bounds on the padded N (PGP_LANE_NP: up to NLANES - 1
+// ascending values separated by commas, default "1024,1536,2560"; "0" keeps
+// every member in lane 0).  A member takes the lane of the number of bounds
+// it exceeds.
 static void lane_bounds(int* out) {
   for (int k = 0; k < NLANES - 1; ++k) out[k] = 0;
   const char* e = getenv("PGP_LANE_NP");
