@@ -42,10 +42,10 @@ import os
 # The serving engine runs each size lane on two streams of its own beside the
 # caller's; the HIP runtime maps a process's streams onto GPU_MAX_HW_QUEUES
 # hardware queues (4 by default), and streams that share a queue serialise.
-# The runtime reads it when it starts, so it is raised to 8 before torch is
-# imported.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# The runtime reads it when it starts, so it is raised to 12 (four lanes'
+# eight streams, the caller's and torch's own) before torch is imported.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 12:
+    os.environ["GPU_MAX_HW_QUEUES"] = "12"
 import sys
 import time
 
