@@ -408,6 +408,20 @@ __device__ __forceinline__ void stash_f32_mmajor(T* d, int pitch, const T (&v)[1
         floatx4{v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]};
 }
 
+// d[m * pitch + 16 kb + q] = bf16(v[q]), q = 0..15, as two 16-byte stores
+// (pitch a multiple of 8): the same LDS image as one 2-byte store per value.
+template <typename T>
+__device__ __forceinline__ void stash_bf16_mmajor(unsigned short* d, int pitch, const T (&v)[16],
+                                                  int m, int kb) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    u16x8 w;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) w[e] = f2bf((float)v[8 * j + e]);
+    *(u16x8*)&d[m * pitch + 16 * kb + 8 * j] = w;
+  }
+}
+
 // Workgroups of one launch are dealt to the 8 XCDs round-robin (workgroup g
 // runs on XCD g % 8).  Renumber them so that each XCD gets one contiguous
 // run of tiles: neighbouring tiles share panel rows, which then stay in that
@@ -442,11 +456,13 @@ __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int 
   T ra[16], rb[16];
   acc_t acc[2][2];
   zero_acc<T>(acc);
+  // the m-major FP32 and BF16 stagings take 16 consecutive k per thread
+  constexpr bool CK = MM || BF16;
   auto fetch = [&](int c) {
     const int64_t c0 = (int64_t)c * TILE;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int k = operand_k<MM>(kb, q);
+      const int k = operand_k<CK>(kb, q);
       ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
       rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
     }
@@ -456,6 +472,9 @@ __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int 
     if constexpr (MM) {
       stash_f32_mmajor(da, PITCH, ra, m, kb);
       stash_f32_mmajor(db, PITCH, rb, m, kb);
+    } else if constexpr (BF16) {
+      stash_bf16_mmajor(da, PITCH, ra, m, kb);
+      stash_bf16_mmajor(db, PITCH, rb, m, kb);
     } else {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
