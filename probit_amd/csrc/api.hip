@@ -79,8 +79,10 @@ constexpr int64_t OFF_PROGRESS_CT = 2560;  // uint64[4] progress words of the wa
 constexpr int64_t HEADER = 4096;
 
 struct WsLayout {
-  int64_t Np, K, L, logdiag, diagval, dinv, vec, symv, csync, total;
+  int64_t Np, K, L, logdiag, diagval, dinv, vec, symv, csync, panel16, total;
 };
+// Columns of the BF16 panel copy: a super-step is at most 16 block columns.
+static inline int64_t panel16_pitch(int64_t Np) { return std::min<int64_t>(Np, 16 * TILE); }
 
 static WsLayout ws_layout(int64_t N, int prec) {
   WsLayout w;
@@ -102,6 +104,11 @@ static WsLayout ws_layout(int64_t N, int prec) {
   off += align256(w.Np * (w.Np / TILE) * 8);
   w.csync = off;
   off += align256(csync_words(w.Np) * 4);
+  w.panel16 = 0;
+  if (prec == PGP_PREC_BF16) {
+    w.panel16 = off;
+    off += align256(w.Np * panel16_pitch(w.Np) * 2);
+  }
   w.total = off;
   return w;
 }
@@ -318,6 +325,8 @@ static Prob make_prob(const pgp_problem_t* pr, const pgp_kernel_t* k, const pgp_
   p.vec = (double*)(ws + L.vec);
   p.symv = (double*)(ws + L.symv);
   p.csync = (uint32_t*)(ws + L.csync);
+  p.panel16 = L.panel16 ? (unsigned short*)(ws + L.panel16) : nullptr;
+  p.panel16_pitch = (int32_t)panel16_pitch(L.Np);
   p.red = (double*)(ws + OFF_RED);
   p.info = (int32_t*)(ws + OFF_INFO);
   p.progress = (uint64_t*)(ws + OFF_PROGRESS);

@@ -439,9 +439,16 @@ __device__ __forceinline__ int xcd_contiguous(int g, int total) {
 // FUSE: the workgroup given factor = true (the tile that becomes the next
 // diagonal block) keeps its updated tile in LDS and factors it at once.
 // Requires kc1 > kc0 (both callers update by at least one block column).
-template <typename T, int BF16, int FUSE>
+// SH (BF16 factors, the outer update): the operand chunks come as BF16 from
+// the super-step's panel copy (P.panel16, row r at (r - p16_row0) * pitch),
+// the same values the stash would convert, so no conversion and half the
+// operand bytes.
+template <typename T, int BF16, int FUSE, bool SH = false>
 __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int kc1, int64_t ri, int64_t rj,
-                                bool factor, const FactorOut& o) {
+                                bool factor, const FactorOut& o,
+                                const unsigned short* p16 = nullptr, int64_t p16_row0 = 0,
+                                int p16_pitch = 0) {
+  static_assert(!SH || BF16, "the panel copy holds BF16 operands");
   auto A = gbl(A_);
   constexpr bool MM = MmajorF32<T, BF16>::value;
   constexpr int PITCH = BF16 ? 72 : MM ? 68 : KP<T>::value;
@@ -458,18 +465,34 @@ __device__ __forceinline__ void syrk_tile_multi(T* A_, int64_t ld, int kc0, int 
   zero_acc<T>(acc);
   // the m-major FP32 and BF16 stagings take 16 consecutive k per thread
   constexpr bool CK = MM || BF16;
+  u16x8 ua[2], ub[2];  // SH: a thread's 16 BF16 operand values of each row
   auto fetch = [&](int c) {
-    const int64_t c0 = (int64_t)c * TILE;
+    if constexpr (SH) {
+      const size_t kofs = (size_t)(c - kc0) * TILE + 16 * kb;
+      const unsigned short* pa = p16 + (size_t)(ri + m - p16_row0) * p16_pitch + kofs;
+      const unsigned short* pb = p16 + (size_t)(rj + m - p16_row0) * p16_pitch + kofs;
+      ua[0] = *(const u16x8*)pa;
+      ua[1] = *(const u16x8*)(pa + 8);
+      ub[0] = *(const u16x8*)pb;
+      ub[1] = *(const u16x8*)(pb + 8);
+    } else {
+      const int64_t c0 = (int64_t)c * TILE;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = operand_k<CK>(kb, q);
-      ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
-      rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
+      for (int q = 0; q < 16; ++q) {
+        const int k = operand_k<CK>(kb, q);
+        ra[q] = A[(size_t)(ri + m) + (size_t)(c0 + k) * ld];
+        rb[q] = A[(size_t)(rj + m) + (size_t)(c0 + k) * ld];
+      }
     }
   };
   fetch(kc0);
   auto stash = [&](LT* da, LT* db) {
-    if constexpr (MM) {
+    if constexpr (SH) {
+      *(u16x8*)&da[m * PITCH + 16 * kb] = ua[0];
+      *(u16x8*)&da[m * PITCH + 16 * kb + 8] = ua[1];
+      *(u16x8*)&db[m * PITCH + 16 * kb] = ub[0];
+      *(u16x8*)&db[m * PITCH + 16 * kb + 8] = ub[1];
+    } else if constexpr (MM) {
       stash_f32_mmajor(da, PITCH, ra, m, kb);
       stash_f32_mmajor(db, PITCH, rb, m, kb);
     } else if constexpr (BF16) {
@@ -606,8 +629,33 @@ __global__ __launch_bounds__(NTHREADS) void k_syrk_outer(Prob* const* probs, Sch
   const FactorOut o{P.L, P.Np, P.dinv + (size_t)s_end * TILE * TILE,
                     P.logdiag ? P.logdiag + c1 : nullptr, P.diagval ? P.diagval + c1 : nullptr,
                     P.info};
-  syrk_tile_multi<T, BF16, FUSE>((T*)P.L, P.Np, s0, s_end, (int64_t)(s_end + ti) * TILE,
-                                     (int64_t)(s_end + tj) * TILE, FUSE && ti == 0 && tj == 0, o);
+  // BF16 factors read the operands from the panel copy (k_panel_bf16)
+  syrk_tile_multi<T, BF16, FUSE, BF16 != 0>(
+      (T*)P.L, P.Np, s0, s_end, (int64_t)(s_end + ti) * TILE, (int64_t)(s_end + tj) * TILE,
+      FUSE && ti == 0 && tj == 0, o, P.panel16, (int64_t)s_end * TILE, P.panel16_pitch);
+}
+
+// BF16 copy of the panel columns [s0, s_end) of the rows below the
+// super-step, for its outer update: one workgroup per 64-row block, each
+// thread converting 16 consecutive columns of one row per block column.
+// The conversion is the one the update's stash applies (f2bf).
+__global__ __launch_bounds__(NTHREADS) void k_panel_bf16(Prob* const* probs, Sched sc, int s0,
+                                                         int s_end) {
+  int bx;
+  const Prob& P = *probs[sched_member(sc, blockIdx.x, bx)];
+  const int nb = P.Np / TILE;
+  if (bx >= nb - s_end) return;
+  const int m = threadIdx.x & 63, kb = threadIdx.x >> 6;
+  const int64_t r = (int64_t)(s_end + bx) * TILE + m;
+  const auto A = gbl((const float*)P.L);
+  unsigned short* dst = P.panel16 + (size_t)(r - (int64_t)s_end * TILE) * P.panel16_pitch;
+  for (int c = s0; c < s_end; ++c) {
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      v[q] = A[(size_t)r + (size_t)((int64_t)c * TILE + 16 * kb + q) * P.Np];
+    stash_bf16_mmajor(dst + (size_t)(c - s0) * TILE, 0, v, 0, kb);
+  }
 }
 
 // ---- one launch per block column ---------------------------------------------
@@ -1323,6 +1371,15 @@ static void potrf_steps(const Batch& b, hipStream_t s, const pgp_tune_entry_t& t
       const int t = std::max(0, b.np(m) / TILE - s_end);
       return t * (t + 1) / 2;
     };
+    if constexpr (BF16 != 0) {
+      // the BF16 copy of the panel rows below the super-step, which the
+      // outer update reads instead of converting at every stash
+      sched_launch(
+          b, [&](int m) { return std::max(0, b.np(m) / TILE - s_end); },
+          [&](Prob* const* dp, const Sched& sc, int n) {
+            hipLaunchKernelGGL(k_panel_bf16, dim3(n), dim3(NTHREADS), 0, s, dp, sc, s0, s_end);
+          });
+    }
     sched_launch(b, outer_tiles, [&](Prob* const* dp, const Sched& sc, int n) {
       if (tsy) timer_begin(TC_SYRK, s);
       if (fuse)
@@ -1342,8 +1399,10 @@ static void potrf_steps(const Batch& b, hipStream_t s, const pgp_tune_entry_t& t
                                       return nn * (nn + 1) * (double)(TILE * G);
                                     }),
                   flops(sc, dp, [&](int nbm) {
+                    // (BF16 factors read the panel from its BF16 copy)
                     const double t = std::max(0, nbm - s_end);
-                    return (t * (t + 1) * TILE * TILE + t * TILE * TILE * G) * es;
+                    return t * (t + 1) * TILE * TILE * es +
+                           t * TILE * TILE * G * (BF16 ? 2.0 : es);
                   }));
       }
     });

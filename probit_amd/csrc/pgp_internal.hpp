@@ -104,6 +104,11 @@ struct Prob {
   // merged column kernel of the Cholesky: 16 (Np/64) tile flags and a
   // workgroup count, zero between launches
   uint32_t* csync;
+  // BF16 factors: BF16 copy of the rows below a super-step of its panel
+  // (row r at (r - first row below) * panel16_pitch), which the super-step's
+  // outer update reads; NULL for FP64 / FP32 factors
+  unsigned short* panel16;
+  int32_t panel16_pitch;
 };
 // words of Prob::csync for padded size Np
 __host__ __device__ constexpr inline int64_t csync_words(int64_t Np) { return 16 * (Np / 64) + 16; }
