@@ -284,3 +284,51 @@ def test_probit_predictive_matches_oracle():
     ref = O.probit_predictive_distributions((0.6, cut), mean, var)
     assert np.max(np.abs(got - ref)) < 1e-14
     assert np.allclose(got.sum(axis=1), 1.0, atol=1e-14)
+
+
+@pytest.mark.parametrize("N", [1600, 700])
+def test_bf16_update_factor_log_determinant(N):
+    """The BF16-update factor of B = I + W^1/2 K W^1/2 (pgp_laplace_set_state in
+    the BF16 mode: super-steps of 8 block columns whose outer updates read the
+    panel's BF16 copy; N = 700 has a single outer update) gives log|B| within
+    BF16-update accuracy of the oracle's FP64 value, and is not the FP32-update
+    factor's: a wrong operand in the BF16 path shows as a non-positive pivot
+    or an O(1) error."""
+    torch, _lib, lib = _setup()
+    from probit_amd.utilities import likelihood_struct
+    rng = np.random.default_rng(N)
+    D = 3
+    X = rng.normal(size=(N, D))
+    cut = np.array([-np.inf, -0.3, 0.4, np.inf])
+    y = np.searchsorted(cut[1:-1], np.sin(X.sum(1)) + 0.5 * rng.normal(size=N)).astype(np.float64)
+    a = 0.05 * rng.normal(size=N)  # a weight away from zero: W varies over the points
+    ks = _kstruct(_lib, "EQ", 1.2, 0.8, D)
+    ls = likelihood_struct(_lib.LIK_PROBIT, (0.5, cut))
+    Xd = torch.as_tensor(X, device="cuda")
+    yd = torch.as_tensor(y, device="cuda")
+    ad = torch.as_tensor(a, device="cuda")
+    out = {}
+    for prec in (0, 1, 2):
+        W = torch.empty(N, dtype=torch.float64, device="cuda")
+        prob = _lib.ProblemT(N, D, 0, Xd.data_ptr(), yd.data_ptr(), None, W.data_ptr(), None)
+        ws = _lib.workspace(lib.pgp_workspace_size(N, D, prec), "cuda")
+        res = _lib.FitResultT()
+        rc = lib.pgp_laplace_set_state(prob, ks, ls, prec, ad.data_ptr(), None, res,
+                                       ws.data_ptr(), ws.numel(), _lib.stream_handle())
+        _lib.check(rc, "pgp_laplace_set_state")
+        torch.cuda.synchronize()
+        assert res.status == 0, (prec, res.status)
+        out[prec] = (res.log_det, W.cpu().numpy())
+    # the oracle's log|B| at the precision the library formed
+    K = O.gram(O.KernelSpec.make(O.EQ, 1.2, 0.8, D), X)
+    s = np.sqrt(np.maximum(out[0][1], 0.0))
+    B = np.eye(N) + s[:, None] * K * s[None, :]
+    ld = 2.0 * np.sum(np.log(np.diag(np.linalg.cholesky(B))))
+    print(f"N {N}: log|B| {ld:.6f}; relative error fp64 {abs(out[0][0] - ld) / abs(ld):.2e} "
+          f"fp32 {abs(out[1][0] - ld) / abs(ld):.2e} bf16 {abs(out[2][0] - ld) / abs(ld):.2e}",
+          flush=True)
+    assert abs(out[0][0] - ld) <= 1e-10 * abs(ld)
+    # measured: FP32 4e-6 / 1e-6, BF16 8e-5 / 2e-4 (N = 1600 / 700)
+    assert abs(out[1][0] - ld) <= 2e-5 * abs(ld)
+    assert abs(out[2][0] - ld) <= 2e-3 * abs(ld), (out[2][0], ld)
+    assert out[2][0] != out[1][0]
