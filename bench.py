@@ -95,6 +95,10 @@ def parse():
                    help="only serve this many requests of the timed set with synchronous steps "
                         "(the same launches on every run: the command the per-kernel profiles "
                         "and counter runs use) and exit")
+    p.add_argument("--factor-only", type=int, default=0, metavar="N",
+                   help="only factor one B of an N-point ordinal-probit problem with FP32 and "
+                        "BF16 updates (twice each, pgp_laplace_set_state) and print the outer "
+                        "update's launch-timer rate")
     p.add_argument("--min-n", type=int, default=0,
                    help="serve only the stream's requests with at least this many points "
                         "(0: the whole stream)")
@@ -304,6 +308,48 @@ def read_timer(lib, _lib, cls):
     return n_l.value, ms.value, fl.value, by.value
 
 
+def factor_only(N, torch, _lib, lib):
+    """The outer update's rate in one large factorisation: B = I + W^1/2 K W^1/2
+    of an ordinal-probit problem (D = 3, EQ, a small random weight) factored by
+    pgp_laplace_set_state with FP32 and with BF16 updates, twice each, the
+    outer-update launch timer on.  One JSON line per factorisation."""
+    import ctypes as C
+
+    from probit_amd.utilities import likelihood_struct
+    D = 3
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(N, D))
+    cut = np.array([-np.inf, -0.3, 0.4, np.inf])
+    y = np.searchsorted(cut[1:-1], np.sin(X.sum(1)) + 0.5 * rng.normal(size=N)).astype(np.float64)
+    a = 0.05 * rng.normal(size=N)
+    ks = _lib.KernelT()
+    ks.kind, ks.dim, ks.variance = _lib.KERNEL_EQ, D, 1.2
+    for d in range(D):
+        ks.lengthscale[d] = 0.8
+    ls = likelihood_struct(_lib.LIK_PROBIT, (0.5, cut))
+    Xd, yd, ad = (torch.as_tensor(v, device="cuda") for v in (X, y, a))
+    for prec in (_lib.PREC_FP32, _lib.PREC_BF16, _lib.PREC_FP32, _lib.PREC_BF16):
+        W = torch.empty(N, dtype=torch.float64, device="cuda")
+        prob = _lib.ProblemT(N, D, 0, Xd.data_ptr(), yd.data_ptr(), None, W.data_ptr(), None)
+        ws = _lib.workspace(lib.pgp_workspace_size(N, D, prec), "cuda")
+        res = _lib.FitResultT()
+        torch.cuda.synchronize()
+        lib.pgp_timer_reset()
+        lib.pgp_timer_enable(1 << _lib.TIMER_SYRK)
+        _lib.check(lib.pgp_laplace_set_state(prob, ks, ls, prec, ad.data_ptr(), None, res,
+                                             ws.data_ptr(), ws.numel(), _lib.stream_handle()),
+                   "pgp_laplace_set_state")
+        torch.cuda.synchronize()
+        lib.pgp_timer_enable(0)
+        n_l, ms, fl, _ = read_timer(lib, _lib, _lib.TIMER_SYRK)
+        print(json.dumps({"factor_only": {
+            "precision": "fp32" if prec == _lib.PREC_FP32 else "bf16", "N": N,
+            "outer_update_launches": n_l, "avg_launch_us": round(1e3 * ms / n_l, 2),
+            "algo_flops_per_launch": fl / n_l,
+            "tflops": round(fl / (ms * 1e-3) / 1e12, 2), "log_det": res.log_det}}), flush=True)
+        del ws
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -337,6 +383,9 @@ def main():
                                                "bytes_written_per_copy": 4 * n, "copies": 3}}))
         return
     lib = _lib.lib()
+    if args.factor_only:
+        factor_only(args.factor_only, torch, _lib, lib)
+        return
 
     stream = data.serving_stream(seed=args.seed, n=min(args.stream, total))
     if args.min_n > 0:
