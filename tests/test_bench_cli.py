@@ -27,7 +27,7 @@ def test_help_lists_the_measurement_options():
     assert r.returncode == 0
     for opt in ("--steps", "--warmup", "--gpus", "--precision", "--min-n", "--plain-sample",
                 "--open-loop", "--cpu-strata", "--cpu-study", "--profile-window",
-                "--copy-calibration", "--kernel-window"):
+                "--copy-calibration", "--kernel-window", "--factor-only"):
         assert opt in r.stdout, opt
 
 
